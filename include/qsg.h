@@ -1,0 +1,128 @@
+/*
+ * qsg.h -- C ABI of the MI355X (gfx950) particle-tracking path.
+ *
+ * One handle = one transport problem whose read-only tables (mesh planes,
+ * cross sections) and particle/tally state live in device memory.  Each
+ * entry point returns QSG_OK or an error code; qsg_last_error() gives the
+ * message (thread-local).  A handle is not re-entrant; distinct handles are
+ * independent.  Host buffers are caller-owned and only accessed during the
+ * call.  No C++ exceptions cross this boundary.
+ *
+ * Reference interfaces replaced (paths relative to the reference's src/):
+ *   qsg_create / qsg_create_from_deck   getParameters  Parameters.cc:80
+ *                                       + initMC       initMC.cc:53
+ *   qsg_cycle_init                      cycleInit      main.cc:96-121
+ *   qsg_cycle_tracking                  cycleTracking  main.cc:138-307
+ *                                       (CycleTrackingGuts, CycleTracking.cc:15)
+ *   qsg_cycle_finalize                  cycleFinalize  main.cc:310-324
+ *                                       (Tallies::CycleFinalize, Tallies.cc:25)
+ *   qsg_get_scalar_flux                 Tallies::_scalarFluxDomain (Tallies.hh:298)
+ *   qsg_set_particles / qsg_get_census  ParticleVaultContainer processing /
+ *                                       processed vaults (ParticleVaultContainer.hh:43-61)
+ *   qsg_destroy                         MonteCarlo::~MonteCarlo  MonteCarlo.cc:113
+ */
+#ifndef QSG_H
+#define QSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSG_OK 0
+#define QSG_ERR_ARG 1      /* bad argument or malformed deck            */
+#define QSG_ERR_DEVICE 2   /* HIP runtime failure                        */
+#define QSG_ERR_CAPACITY 3 /* a particle buffer would overflow           */
+#define QSG_ERR_STATE 4    /* call out of order (e.g. finalize twice)    */
+#define QSG_ERR_NUMERIC 5  /* runaway history / production overflow      */
+
+typedef struct qsg_problem_s* qsg_problem;
+
+/* Balance tallies of one cycle (Tallies.hh:36-100 Balance). */
+typedef struct qsg_balance {
+    uint64_t start, source, rr, split;
+    uint64_t absorb, scatter, fission, produce;
+    uint64_t collision, escape, census, num_segments;
+    uint64_t end;
+} qsg_balance;
+
+/* Size information of a problem. */
+typedef struct qsg_info {
+    int32_t nx, ny, nz, n_groups;
+    int32_t n_materials, n_steps;
+    uint64_t n_cells;
+    uint64_t n_particles_target;
+    uint64_t source_per_cycle;
+    uint64_t particle_capacity;
+    uint64_t device_bytes;
+    double dt;
+    double source_weight;
+    int32_t cycle;      /* cycles completed so far */
+    int32_t lds_tables; /* 1 if the cross-section tables are staged in LDS */
+} qsg_info;
+
+/* Particle state in structure-of-arrays form (host pointers, n entries). */
+typedef struct qsg_particles {
+    double *x, *y, *z, *vx, *vy, *vz;
+    double *energy, *weight, *time_to_census, *num_mean_free_paths;
+    uint64_t* seed;
+    int32_t* cell; /* global cell id i + nx*(j + ny*k) */
+} qsg_particles;
+
+/* Kernel launch shape of the tracking pass (chosen by the autotuner). */
+typedef struct qsg_tuning {
+    int32_t block_size;   /* threads per workgroup, 64..256, multiple of 64 */
+    int32_t waves_per_cu; /* persistent waves launched per compute unit    */
+    int32_t refill_min;   /* idle lanes (1..64) that trigger a wave refill  */
+    int32_t chunk;        /* queue entries a wave claims at a time          */
+} qsg_tuning;
+
+const char* qsg_last_error(void);
+
+/* argv-style construction: same flags and deck grammar as the qs driver. */
+int qsg_create(int argc, const char* const* argv, int device, qsg_problem* out);
+/* Deck given as text (optional extra argv applied first, like the CLI). */
+int qsg_create_from_deck(const char* deck_text, int argc, const char* const* argv, int device,
+                         qsg_problem* out);
+int qsg_destroy(qsg_problem p);
+
+int qsg_get_info(qsg_problem p, qsg_info* info);
+int qsg_set_tuning(qsg_problem p, const qsg_tuning* t);
+int qsg_get_tuning(qsg_problem p, qsg_tuning* t);
+
+/* Per-cycle phases, in order. */
+int qsg_cycle_init(qsg_problem p);
+int qsg_cycle_tracking(qsg_problem p);
+/* Writes this cycle's balance and scalar-flux sum, then zeroes the flux. */
+int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum);
+
+/* Runs n cycles; per-cycle results into arrays of length n (may be NULL).
+ * tracking_seconds (may be NULL) receives the device time of the tracking
+ * phase of each cycle. */
+int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums,
+            double* tracking_seconds);
+
+/* Scalar flux [n_cells][n_groups] accumulated so far in this cycle. */
+int qsg_get_scalar_flux(qsg_problem p, double* out, uint64_t n);
+
+/* Replace the processing population (after qsg_cycle_init, before
+ * qsg_cycle_tracking).  Used to track a given set of particles. */
+int qsg_set_particles(qsg_problem p, const qsg_particles* in, uint64_t n);
+/* Number of census particles after qsg_cycle_tracking. */
+int qsg_get_census_count(qsg_problem p, uint64_t* n);
+int qsg_get_census(qsg_problem p, qsg_particles* out, uint64_t n);
+/* Processing population after qsg_cycle_init. */
+int qsg_get_processing_count(qsg_problem p, uint64_t* n);
+int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n);
+
+/* Device FP64 builtins on host arrays (rounding checks).
+ * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a) */
+int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSG_H */

@@ -1,0 +1,222 @@
+// Test infrastructure only: a driver for the UNMODIFIED Quicksilver CPU
+// sources (compiled in place from the reference tree by oracle/Makefile).
+//
+// The reference's own driver (src/main.cc) cannot be compiled here because it
+// includes build-generated headers (git_hash.hh, git_vers.hh, main.cc:26-27).
+// This file is a small single-rank driver written for this repository.  It
+// calls the reference's public functions in the order main.cc uses them:
+//   getParameters            Parameters.cc:80
+//   initMC                   initMC.cc:53
+//   per step: cycle_init  -> main.cc:96-121   (source, population control)
+//             cycle_track -> main.cc:138-307  (tracking loop, 1 rank, no MPI)
+//             cycle_final -> main.cc:310-324  (tally reduction / print)
+// and adds full-precision output so the scalar flux can be checked to 1e-10
+// (the reference prints it with %14e, Tallies.cc:136-141).
+//
+// Extra output, one line per cycle on stdout:
+//   QSREF <cycle> <start> <source> <rr> <split> <absorb> <scatter> <fission>
+//         <produce> <collision> <escape> <census> <num_seg> <end>
+//         <flux_sum %.17e> <tracking_seconds>
+// and at the end
+//   QSREF_FOM <total segments> <total tracking seconds> <segments/s> <threads>
+//
+// Environment (optional):
+//   QSREF_FLUX_DUMP=path   append, per cycle, the scalar flux of every
+//                          (global cell id, group) as raw float64 in
+//                          global-cell-major order (requires one domain).
+//   QSREF_CENSUS_DUMP=path write the census particles of the last cycle as
+//                          text: seed x y z vx vy vz E w tc nmfp cell
+#include <chrono>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include <algorithm>
+
+#include "Parameters.hh"
+#include "utilsMpi.hh"
+#include "MonteCarlo.hh"
+#include "initMC.hh"
+#include "Tallies.hh"
+#include "PopulationControl.hh"
+#include "ParticleVaultContainer.hh"
+#include "ParticleVault.hh"
+#include "MC_Particle_Buffer.hh"
+#include "MC_Processor_Info.hh"
+#include "MC_Time_Info.hh"
+#include "MC_SourceNow.hh"
+#include "MC_Domain.hh"
+#include "CycleTracking.hh"
+#include "SendQueue.hh"
+#include "MC_Fast_Timer.hh"
+
+#ifdef HAVE_OPENMP
+#include <omp.h>
+#endif
+
+MonteCarlo* mcco = NULL;  // declared extern in Globals.hh
+
+namespace {
+
+void cycle_init(MonteCarlo* mc, bool loadBalance)
+{
+    mc->clearCrossSectionCache();
+    mc->_tallies->CycleInitialize(mc);
+    ParticleVaultContainer& pvc = *mc->_particleVaultContainer;
+    pvc.swapProcessingProcessedVaults();
+    pvc.collapseProcessed();
+    pvc.collapseProcessing();
+    mc->_tallies->_balanceTask[0]._start = pvc.sizeProcessing();
+    mc->particle_buffer->Initialize();
+    MC_SourceNow(mc);
+    PopulationControl(mc, loadBalance);
+    RouletteLowWeightParticles(mc);
+}
+
+// Single rank, single process: no particle leaves the rank, so the MPI
+// exchange of main.cc:243-275 reduces to clearing the processed vault and
+// folding the extra (secondary) vaults back in; the loop ends when no
+// processing particles remain (MC_Particle_Buffer.cc:298-312).
+void cycle_track(MonteCarlo* mc)
+{
+    ParticleVaultContainer& pvc = *mc->_particleVaultContainer;
+    bool done = false;
+    while (!done)
+    {
+        for (uint64_t v = 0; v < pvc.processingSize(); v++)
+        {
+            uint64_t pv = pvc.getFirstEmptyProcessedVault();
+            ParticleVault* processing = pvc.getTaskProcessingVault(v);
+            ParticleVault* processed = pvc.getTaskProcessedVault(pv);
+            int n = (int)processing->size();
+            if (n != 0)
+            {
+#ifdef HAVE_OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+                for (int i = 0; i < n; i++)
+                    CycleTrackingGuts(mc, i, processing, processed);
+            }
+            processing->clear();
+            pvc.getSendQueue()->clear();
+            pvc.cleanExtraVaults();
+        }
+        pvc.collapseProcessing();
+        pvc.collapseProcessed();
+        done = (pvc.sizeProcessing() == 0);
+    }
+}
+
+double flux_sum(MonteCarlo* mc)
+{
+    // Same summation order as Tallies::ScalarFluxSum (Tallies.cc:146-171).
+    double s = 0.0;
+    Tallies* t = mc->_tallies;
+    for (int d = 0; d < (int)t->_scalarFluxDomain.size(); d++)
+        for (int r = 0; r < t->GetNumFluxReplications(); r++)
+        {
+            ScalarFluxTask& task = t->_scalarFluxDomain[d]._task[r];
+            for (int c = 0; c < (int)task._cell.size(); c++)
+                for (int g = 0; g < task._cell[c].size(); g++)
+                    s += task._cell[c]._group[g];
+        }
+    return s;
+}
+
+void dump_flux(MonteCarlo* mc, FILE* f)
+{
+    Tallies* t = mc->_tallies;
+    if (t->_scalarFluxDomain.size() != 1) { fprintf(stderr, "flux dump needs one domain\n"); exit(3); }
+    ScalarFluxTask& task = t->_scalarFluxDomain[0]._task[0];
+    int nc = (int)task._cell.size();
+    int ng = task._cell[0].size();
+    std::vector<double> out((size_t)nc * ng, 0.0);
+    MC_Domain& dom = mc->domain[0];
+    for (int c = 0; c < nc; c++)
+    {
+        uint64_t gid = dom.cell_state[c]._id >> 32;
+        for (int g = 0; g < ng; g++)
+        {
+            double v = 0.0;
+            for (int r = 0; r < t->GetNumFluxReplications(); r++)
+                v += t->_scalarFluxDomain[0]._task[r]._cell[c]._group[g];
+            out[gid * ng + g] = v;
+        }
+    }
+    fwrite(out.data(), sizeof(double), out.size(), f);
+}
+
+void dump_census(MonteCarlo* mc, const char* path)
+{
+    FILE* f = fopen(path, "w");
+    if (!f) return;
+    ParticleVaultContainer& pvc = *mc->_particleVaultContainer;
+    // after cycle_track the census particles sit in the processed vaults
+    for (uint64_t v = 0; v < pvc.processedSize(); v++)
+    {
+        ParticleVault* pv = pvc.getTaskProcessedVault(v);
+        for (size_t i = 0; i < pv->size(); i++)
+        {
+            const MC_Base_Particle& p = (*pv)[i];
+            fprintf(f, "%" PRIu64 " %.17e %.17e %.17e %.17e %.17e %.17e %.17e %.17e %.17e %.17e %d\n",
+                    p.random_number_seed, p.coordinate.x, p.coordinate.y, p.coordinate.z,
+                    p.velocity.x, p.velocity.y, p.velocity.z, p.kinetic_energy, p.weight,
+                    p.time_to_census, p.num_mean_free_paths,
+                    (int)(mc->domain[p.domain].cell_state[p.cell]._id >> 32));
+        }
+    }
+    fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    mpiInit(&argc, &argv);
+    Parameters params = getParameters(argc, argv);
+    mcco = initMC(params);
+
+    const char* fluxPath = getenv("QSREF_FLUX_DUMP");
+    const char* censusPath = getenv("QSREF_CENSUS_DUMP");
+    FILE* fluxFile = fluxPath ? fopen(fluxPath, "wb") : NULL;
+
+    int threads = 1;
+#ifdef HAVE_OPENMP
+    threads = omp_get_max_threads();
+#endif
+    const int nSteps = params.simulationParams.nSteps;
+    uint64_t totalSeg = 0;
+    double totalTrack = 0.0;
+    for (int step = 0; step < nSteps; step++)
+    {
+        cycle_init(mcco, params.simulationParams.loadBalance != 0);
+        auto t0 = std::chrono::steady_clock::now();
+        cycle_track(mcco);
+        auto t1 = std::chrono::steady_clock::now();
+        double secs = std::chrono::duration<double>(t1 - t0).count();
+
+        Balance& b = mcco->_tallies->_balanceTask[0];
+        b._end = mcco->_particleVaultContainer->sizeProcessed();
+        double fs = flux_sum(mcco);
+        printf("QSREF %d %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64
+               " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64 " %" PRIu64
+               " %.17e %.6e\n",
+               step, b._start, b._source, b._rr, b._split, b._absorb, b._scatter, b._fission,
+               b._produce, b._collision, b._escape, b._census, b._numSegments, b._end, fs, secs);
+        totalSeg += b._numSegments;
+        totalTrack += secs;
+        if (fluxFile) dump_flux(mcco, fluxFile);
+        if (censusPath && step == nSteps - 1) dump_census(mcco, censusPath);
+
+        mcco->_tallies->CycleFinalize(mcco);
+        mcco->time_info->cycle++;
+        mcco->particle_buffer->Free_Memory();
+    }
+    if (fluxFile) fclose(fluxFile);
+    printf("QSREF_FOM %" PRIu64 " %.6e %.6e %d\n", totalSeg, totalTrack,
+           totalTrack > 0 ? (double)totalSeg / totalTrack : 0.0, threads);
+    fflush(stdout);
+    delete mcco;
+    mpiFinalize();
+    return 0;
+}

@@ -1,0 +1,274 @@
+"""MI355X (gfx950) particle-tracking path of Quicksilver, behind a C ABI.
+
+The package is a thin ctypes view of ``libqsgpu.so`` (include/qsg.h).  The
+names mirror the reference driver: ``cycle_init`` / ``cycle_tracking`` /
+``cycle_finalize`` (main.cc:96-324), ``run`` for a whole deck.  There is no
+CPU fallback: if the library is missing or no GPU is visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqsgpu.so")
+QS_EXE = os.path.join(_HERE, "qs")
+
+QSG_OK = 0
+ERRORS = {1: "ARG", 2: "DEVICE", 3: "CAPACITY", 4: "STATE", 5: "NUMERIC"}
+
+BALANCE_FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce",
+                  "collision", "escape", "census", "num_segments", "end"]
+
+#: every entry point declared in include/qsg.h
+EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy", "qsg_get_info",
+           "qsg_set_tuning", "qsg_get_tuning", "qsg_cycle_init", "qsg_cycle_tracking",
+           "qsg_cycle_finalize", "qsg_run", "qsg_get_scalar_flux", "qsg_set_particles",
+           "qsg_get_census_count", "qsg_get_census", "qsg_get_processing_count",
+           "qsg_get_processing", "qsg_math_probe"]
+
+
+class QsgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qsg error {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Balance(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in BALANCE_FIELDS]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f in BALANCE_FIELDS}
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("n_groups", ctypes.c_int32), ("n_materials", ctypes.c_int32),
+                ("n_steps", ctypes.c_int32), ("n_cells", ctypes.c_uint64),
+                ("n_particles_target", ctypes.c_uint64), ("source_per_cycle", ctypes.c_uint64),
+                ("particle_capacity", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("dt", ctypes.c_double), ("source_weight", ctypes.c_double),
+                ("cycle", ctypes.c_int32), ("lds_tables", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("refill_min", ctypes.c_int32), ("chunk", ctypes.c_int32)]
+
+
+_P = ctypes.POINTER
+_D = _P(ctypes.c_double)
+
+
+class ParticlesC(ctypes.Structure):
+    _fields_ = [("x", _D), ("y", _D), ("z", _D), ("vx", _D), ("vy", _D), ("vz", _D),
+                ("energy", _D), ("weight", _D), ("time_to_census", _D),
+                ("num_mean_free_paths", _D), ("seed", _P(ctypes.c_uint64)),
+                ("cell", _P(ctypes.c_int32))]
+
+
+PARTICLE_FLOAT_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "energy", "weight", "time_to_census",
+                         "num_mean_free_paths"]
+
+
+@dataclass
+class ParticleSet:
+    """Host copy of particle state (structure of arrays)."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    energy: np.ndarray
+    weight: np.ndarray
+    time_to_census: np.ndarray
+    num_mean_free_paths: np.ndarray
+    seed: np.ndarray
+    cell: np.ndarray
+
+    @classmethod
+    def empty(cls, n: int) -> "ParticleSet":
+        f = {k: np.zeros(n, dtype=np.float64) for k in PARTICLE_FLOAT_FIELDS}
+        return cls(**f, seed=np.zeros(n, dtype=np.uint64), cell=np.zeros(n, dtype=np.int32))
+
+    def __len__(self):
+        return len(self.seed)
+
+    def c_view(self) -> ParticlesC:
+        for k in PARTICLE_FLOAT_FIELDS:
+            a = getattr(self, k)
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                setattr(self, k, np.ascontiguousarray(a, dtype=np.float64))
+        self.seed = np.ascontiguousarray(self.seed, dtype=np.uint64)
+        self.cell = np.ascontiguousarray(self.cell, dtype=np.int32)
+        args = {k: getattr(self, k).ctypes.data_as(_D) for k in PARTICLE_FLOAT_FIELDS}
+        return ParticlesC(**args, seed=self.seed.ctypes.data_as(_P(ctypes.c_uint64)),
+                          cell=self.cell.ctypes.data_as(_P(ctypes.c_int32)))
+
+    def state(self) -> np.ndarray:
+        """[n, 10] float64: x y z vx vy vz E w tc nmfp."""
+        return np.stack([getattr(self, k) for k in PARTICLE_FLOAT_FIELDS], axis=1)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libqsgpu.so (built by ``make -C quicksilver_amd``); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    i, c, vp = ctypes.c_int, ctypes.c_char_p, ctypes.c_void_p
+    argv_t = _P(c)
+    L.qsg_last_error.restype = c
+    L.qsg_create.argtypes = [i, argv_t, i, _P(vp)]
+    L.qsg_create_from_deck.argtypes = [c, i, argv_t, i, _P(vp)]
+    L.qsg_destroy.argtypes = [vp]
+    L.qsg_get_info.argtypes = [vp, _P(Info)]
+    L.qsg_set_tuning.argtypes = [vp, _P(Tuning)]
+    L.qsg_get_tuning.argtypes = [vp, _P(Tuning)]
+    L.qsg_cycle_init.argtypes = [vp]
+    L.qsg_cycle_tracking.argtypes = [vp]
+    L.qsg_cycle_finalize.argtypes = [vp, _P(Balance), _D]
+    L.qsg_run.argtypes = [vp, i, _P(Balance), _D, _D]
+    L.qsg_get_scalar_flux.argtypes = [vp, _D, ctypes.c_uint64]
+    L.qsg_set_particles.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
+    L.qsg_get_census_count.argtypes = [vp, _P(ctypes.c_uint64)]
+    L.qsg_get_census.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
+    L.qsg_get_processing_count.argtypes = [vp, _P(ctypes.c_uint64)]
+    L.qsg_get_processing.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
+    L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != QSG_OK:
+        raise QsgError(rc, lib().qsg_last_error().decode())
+
+
+def _argv(args: Sequence[str]):
+    arr = (ctypes.c_char_p * (len(args) + 1))()
+    for k, a in enumerate(args):
+        arr[k] = a.encode()
+    arr[len(args)] = None
+    return arr
+
+
+class Problem:
+    """One transport problem resident on a GPU (a qsg_problem handle)."""
+
+    def __init__(self, deck_text: Optional[str] = None, args: Sequence[str] = (), device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        argv = ["qs", *args]
+        if deck_text is None:
+            _check(L.qsg_create(len(argv), _argv(argv), device, ctypes.byref(h)))
+        else:
+            _check(L.qsg_create_from_deck(deck_text.encode(), len(argv), _argv(argv), device,
+                                          ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_file(cls, path: str, device: int = 0, extra_args: Sequence[str] = ()):
+        return cls(None, ["-i", path, *extra_args], device)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().qsg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- introspection / tuning
+    def info(self) -> Info:
+        inf = Info()
+        _check(lib().qsg_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().qsg_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
+        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+        _check(lib().qsg_set_tuning(self._h, ctypes.byref(t)))
+
+    # ---- per-cycle phases (main.cc:55-66)
+    def cycle_init(self):
+        _check(lib().qsg_cycle_init(self._h))
+
+    def cycle_tracking(self):
+        _check(lib().qsg_cycle_tracking(self._h))
+
+    def cycle_finalize(self):
+        b = Balance()
+        fs = ctypes.c_double()
+        _check(lib().qsg_cycle_finalize(self._h, ctypes.byref(b), ctypes.byref(fs)))
+        return b.as_dict(), fs.value
+
+    def run(self, n_cycles: Optional[int] = None):
+        """Runs n cycles (default: the deck's nSteps); returns a list of
+        (balance dict, flux sum, tracking seconds)."""
+        n = self.info().n_steps if n_cycles is None else n_cycles
+        bal = (Balance * max(n, 1))()
+        flux = (ctypes.c_double * max(n, 1))()
+        secs = (ctypes.c_double * max(n, 1))()
+        _check(lib().qsg_run(self._h, n, bal, flux, secs))
+        return [(bal[k].as_dict(), flux[k], secs[k]) for k in range(n)]
+
+    def scalar_flux(self) -> np.ndarray:
+        inf = self.info()
+        out = np.zeros((inf.n_cells, inf.n_groups), dtype=np.float64)
+        _check(lib().qsg_get_scalar_flux(self._h, out.ctypes.data_as(_D), out.size))
+        return out
+
+    # ---- particle state
+    def set_particles(self, ps: ParticleSet):
+        v = ps.c_view()
+        _check(lib().qsg_set_particles(self._h, ctypes.byref(v), len(ps)))
+
+    def census(self) -> ParticleSet:
+        n = ctypes.c_uint64()
+        _check(lib().qsg_get_census_count(self._h, ctypes.byref(n)))
+        ps = ParticleSet.empty(n.value)
+        v = ps.c_view()
+        _check(lib().qsg_get_census(self._h, ctypes.byref(v), n.value))
+        return ps
+
+    def processing(self) -> ParticleSet:
+        n = ctypes.c_uint64()
+        _check(lib().qsg_get_processing_count(self._h, ctypes.byref(n)))
+        ps = ParticleSet.empty(n.value)
+        v = ps.c_view()
+        _check(lib().qsg_get_processing(self._h, ctypes.byref(v), n.value))
+        return ps
+
+
+def math_probe(op: int, a: np.ndarray, b: Optional[np.ndarray] = None, device: int = 0) -> np.ndarray:
+    """Evaluates one FP64 builtin on the device (see qsg_math_probe)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(a if b is None else b, dtype=np.float64)
+    out = np.zeros_like(a)
+    _check(lib().qsg_math_probe(op, a.ctypes.data_as(_D), b.ctypes.data_as(_D), out.ctypes.data_as(_D),
+                                a.size, device))
+    return out

@@ -1,0 +1,1060 @@
+// gfx950 kernels of the per-cycle particle-tracking path.
+//
+// Arithmetic is FP64 throughout and compiled with -ffp-contract=off so every
+// expression rounds exactly like the reference's non-FMA x86-64 build; the
+// order of operations inside each expression follows the reference line
+// cited next to it.  Integer RNG streams are bit-exact.  log/sin/cos come
+// from the device math library (within 1 ulp of glibc).
+//
+// Kernels
+//   k_source       one thread per source particle      MC_SourceNow.cc:68-151
+//   k_population   one thread per particle              PopulationControl.cc:71-171
+//   k_track        persistent waves over a work queue   CycleTracking.cc:15-119
+//   k_flux_*       deterministic flux sum + reset       Tallies.cc:95,146-171
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "qs_device.h"
+
+namespace qs {
+namespace {
+
+constexpr double kHuge = 1.0e+75;   // PhysicalConstants::_hugeDouble
+constexpr double kSmall = 1.0e-10;  // _smallDouble
+constexpr double kTiny = 1.0e-13;   // _tinyDouble
+constexpr double kRestMass = 9.395656981095e+2;
+constexpr double kLight = 2.99792458e+10;
+constexpr double kPi = 3.1415926535897932;
+constexpr uint32_t kMaxSegmentsPerHistory = 1u << 26;
+
+enum { kScatter = 1, kAbsorption = 2, kFission = 3 };
+enum { kEscape = 1, kReflect = 2 };
+
+// ---------------------------------------------------------------- RNG ----
+// 64-bit LCG (MC_RNG_State.hh:23-32).
+__device__ __forceinline__ double rng_sample(uint64_t& s) {
+    s = 2862933555777941757ULL * s + 3037000493ULL;
+    return 5.4210108624275222e-20 * (double)s;
+}
+
+// Numerical Recipes psdes hash with two rounds (MC_RNG_State.cc:27-98).
+__device__ __forceinline__ uint64_t hash_state(uint64_t in) {
+    uint32_t lword = (uint32_t)(in >> 32);
+    uint32_t irword = (uint32_t)(in & 0xffffffffu);
+    const uint32_t c1[2] = {0xbaa96887u, 0x1e17d32cu};
+    const uint32_t c2[2] = {0x4b0f3b58u, 0xe874f0c3u};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        uint32_t iswap = irword;
+        uint32_t ia = iswap ^ c1[i];
+        uint32_t lo = ia & 0xffffu;
+        uint32_t hi = ia >> 16;
+        uint32_t ib = lo * lo + ~(hi * hi);
+        ia = (ib >> 16) | ((ib & 0xffffu) << 16);
+        irword = lword ^ ((ia ^ c2[i]) + lo * hi);
+        lword = iswap;
+    }
+    return ((uint64_t)lword << 32) | (uint64_t)irword;
+}
+
+// rngSpawn_Random_Number_Seed (MC_RNG_State.cc:107-113)
+__device__ __forceinline__ uint64_t rng_spawn(uint64_t& parent) {
+    uint64_t child = hash_state(parent);
+    rng_sample(parent);
+    return child;
+}
+
+// NuclearData::getEnergyGroup (NuclearData.cc:208-228).  The reference
+// returns G for E > eMax, one past the last table row; tables here have G
+// rows, so that case is clamped to G-1.
+__device__ __forceinline__ int energy_group(const double* e, int G, double E) {
+    if (E <= e[0]) return 0;
+    if (E > e[G]) return G - 1;
+    int high = G, low = 0;
+    while (high != low + 1) {
+        int mid = (high + low) / 2;
+        if (E < e[mid]) high = mid;
+        else low = mid;
+    }
+    return low;
+}
+
+// --------------------------------------------------------- geometry ----
+// Per-facet node codes: for each of the facet's 3 nodes, 2 bits per axis
+// selecting {0: i*dx, 1: (i+1)*dx, 2: i*dx + dx/2}.  Built from the corner
+// tuples (GlobalFccGrid.cc:47-69) and the facet node triples
+// (MC_Domain.cc:42-47) by facetNodeCodes() on the host side of this file.
+__constant__ uint32_t c_facetCode[24];
+
+struct Cell9 {
+    double X0, X1, Xh, Y0, Y1, Yh, Z0, Z1, Zh;
+};
+
+__device__ __forceinline__ Cell9 cell_nodes(const DevProblem& P, int i, int j, int k) {
+    Cell9 c;
+    c.X0 = (double)i * P.dx;
+    c.X1 = (double)(i + 1) * P.dx;
+    c.Xh = c.X0 + P.hdx;
+    c.Y0 = (double)j * P.dy;
+    c.Y1 = (double)(j + 1) * P.dy;
+    c.Yh = c.Y0 + P.hdy;
+    c.Z0 = (double)k * P.dz;
+    c.Z1 = (double)(k + 1) * P.dz;
+    c.Zh = c.Z0 + P.hdz;
+    return c;
+}
+
+__device__ __forceinline__ double sel3(uint32_t code, double v0, double v1, double vh) {
+    return code == 0 ? v0 : (code == 1 ? v1 : vh);
+}
+
+struct Node3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ Node3 facet_node(const Cell9& c, uint32_t code, int m) {
+    uint32_t b = code >> (6 * m);
+    Node3 n;
+    n.x = sel3(b & 3u, c.X0, c.X1, c.Xh);
+    n.y = sel3((b >> 2) & 3u, c.Y0, c.Y1, c.Yh);
+    n.z = sel3((b >> 4) & 3u, c.Z0, c.Z1, c.Zh);
+    return n;
+}
+
+// MCT_Nearest_Facet_3D_G_Distance_To_Segment (MCT.cc:280-395) for a facet
+// whose outward normal has a positive dot product with the direction.  The
+// reference's three projection branches (|C|>0.5, |B|>0.5, |A|>0.5) are
+// folded into a per-lane choice of the projected axes (u,v), which keeps the
+// wave converged; each branch checks the bounding box on exactly the two
+// projected axes, so only the selection differs, not the arithmetic.
+__device__ __forceinline__ double distance_to_facet(double plane_tol, double dot, double A, double B,
+                                                    double C, double D, const Node3& p0,
+                                                    const Node3& p1, const Node3& p2, double x,
+                                                    double y, double z, double a, double b, double g) {
+    const double tol = 1e-9;
+    double numerator = -1.0 * (A * x + B * y + C * z + D);
+    if (numerator < 0.0 && numerator * numerator > plane_tol) return kHuge;
+    double distance = numerator / dot;
+    double ix = x + distance * a;
+    double iy = y + distance * b;
+    double iz = z + distance * g;
+
+    int axis = (C < -0.5 || C > 0.5) ? 2 : ((B < -0.5 || B > 0.5) ? 1 : ((A < -0.5 || A > 0.5) ? 0 : -1));
+    if (axis < 0) return kHuge;
+    // axis 2: (u,v) = (x,y); axis 1: (z,x); axis 0: (y,z)
+    double p0u = axis == 2 ? p0.x : (axis == 1 ? p0.z : p0.y);
+    double p1u = axis == 2 ? p1.x : (axis == 1 ? p1.z : p1.y);
+    double p2u = axis == 2 ? p2.x : (axis == 1 ? p2.z : p2.y);
+    double iu = axis == 2 ? ix : (axis == 1 ? iz : iy);
+    double p0v = axis == 2 ? p0.y : (axis == 1 ? p0.x : p0.z);
+    double p1v = axis == 2 ? p1.y : (axis == 1 ? p1.x : p1.z);
+    double p2v = axis == 2 ? p2.y : (axis == 1 ? p2.x : p2.z);
+    double iv = axis == 2 ? iy : (axis == 1 ? ix : iz);
+
+    double iuP = iu + tol, iuM = iu - tol, ivP = iv + tol, ivM = iv - tol;
+    if (p0u > iuP && p1u > iuP && p2u > iuP) return kHuge;
+    if (p0u < iuM && p1u < iuM && p2u < iuM) return kHuge;
+    if (p0v > ivP && p1v > ivP && p2v > ivP) return kHuge;
+    if (p0v < ivM && p1v < ivM && p2v < ivM) return kHuge;
+
+    double cross1 = (p1u - p0u) * (iv - p0v) - (p1v - p0v) * (iu - p0u);
+    double cross2 = (p2u - p1u) * (iv - p1v) - (p2v - p1v) * (iu - p1u);
+    double cross0 = (p0u - p2u) * (iv - p2v) - (p0v - p2v) * (iu - p2u);
+    double s = cross0 + cross1 + cross2;
+    double ctol = 1e-9 * (s < 0 ? -s : s);
+    if ((cross0 > -ctol && cross1 > -ctol && cross2 > -ctol) ||
+        (cross0 < ctol && cross1 < ctol && cross2 < ctol))
+        return distance;
+    return kHuge;
+}
+
+struct Nearest {
+    double d;
+    int f;
+};
+
+// MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G + both Find_Nearest routines
+// (MCT.cc:87-137, 436-536, 550-621).  Facets facing away from the direction
+// (dot <= 0) keep distance kHuge; the candidates are visited in increasing
+// facet order and the skipped ranges are merged in with the same `<=` rule,
+// so ties and the "no positive distance" fallback resolve exactly as in the
+// reference.  May move (x,y,z) toward the cell centre and retry.
+__device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell,
+                                                 const Cell9& c9, double& x, double& y, double& z,
+                                                 double a, double b, double g) {
+    uint32_t cand = 0;
+#pragma unroll
+    for (int f = 0; f < 24; f++) {
+        double A = pl[4 * f + 0], B = pl[4 * f + 1], C = pl[4 * f + 2];
+        double dot = A * a + B * b + C * g;
+        if (dot > 0.0) cand |= 1u << f;
+    }
+    int iteration = 0;
+    double moveFactor = 0.5 * kSmall;
+    Nearest nf;
+    for (;;) {
+        double planeTol = 1e-16 * (x * x + y * y + z * z);
+        double nearD = 1e80;
+        int nearF = 0;
+        double negD = -kHuge;
+        int negF = 0;
+        int prev = -1;
+        uint32_t m = cand;
+        while (m) {
+            int f = __builtin_ctz(m);
+            m &= m - 1;
+            uint32_t gapMask = ~cand & (((1u << f) - 1u) & ~((prev >= 0) ? ((2u << prev) - 1u) : 0u));
+            if (gapMask && kHuge <= nearD) {
+                nearD = kHuge;
+                nearF = 31 - __builtin_clz(gapMask);
+            }
+            const double* p = pl + 4 * f;
+            double A = p[0], B = p[1], C = p[2], D = p[3];
+            double dot = A * a + B * b + C * g;
+            uint32_t code = c_facetCode[f];
+            Node3 n0 = facet_node(c9, code, 0);
+            Node3 n1 = facet_node(c9, code, 1);
+            Node3 n2 = facet_node(c9, code, 2);
+            double d = distance_to_facet(planeTol, dot, A, B, C, D, n0, n1, n2, x, y, z, a, b, g);
+            if (d > 0.0) {
+                if (d <= nearD) {
+                    nearD = d;
+                    nearF = f;
+                }
+            } else if (d > negD) {
+                negD = d;
+                negF = f;
+            }
+            prev = f;
+        }
+        {
+            uint32_t gapMask = ~cand & (0xFFFFFFu & ~((prev >= 0) ? ((2u << prev) - 1u) : 0u));
+            if (gapMask && kHuge <= nearD) {
+                nearD = kHuge;
+                nearF = 31 - __builtin_clz(gapMask);
+            }
+        }
+        if (nearD == kHuge && negD != -kHuge) {
+            nearD = negD;
+            nearF = negF;
+        }
+        nf.d = nearD;
+        nf.f = nearF;
+        if (nearD != kHuge) break;
+        // no facet bounds the particle: nudge toward the cell centre
+        const double* cc = P.cellCenter + 3 * (size_t)cell;
+        x += moveFactor * (cc[0] - x);
+        y += moveFactor * (cc[1] - y);
+        z += moveFactor * (cc[2] - z);
+        iteration++;
+        moveFactor *= 2.0;
+        if (moveFactor > 1.0e-2) moveFactor = 1.0e-2;
+        if (iteration == 10000) break;
+    }
+    if (nf.d < 0) nf.d = 0;
+    return nf;
+}
+
+// DirectionCosine::Rotate3DVector (DirectionCosine.hh:123-146)
+__device__ __forceinline__ void rotate3d(double& a, double& b, double& g, double sinT, double cosT,
+                                         double sinP, double cosP) {
+    double cos_theta = g;
+    double sin_theta = sqrt((1.0 - (cos_theta * cos_theta)));
+    double cos_phi, sin_phi;
+    if (sin_theta < 1e-6) {
+        cos_phi = 1.0;
+        sin_phi = 0.0;
+    } else {
+        cos_phi = a / sin_theta;
+        sin_phi = b / sin_theta;
+    }
+    double na = cos_theta * cos_phi * (sinT * cosP) - sin_phi * (sinT * sinP) + sin_theta * cos_phi * cosT;
+    double nb = cos_theta * sin_phi * (sinT * cosP) + cos_phi * (sinT * sinP) + sin_theta * sin_phi * cosT;
+    double ng = -sin_theta * (sinT * cosP) + cos_theta * cosT;
+    a = na;
+    b = nb;
+    g = ng;
+}
+
+struct Lane {
+    double x, y, z;
+    double a, b, g;     // direction cosines
+    double vx, vy, vz;  // velocity
+    double speed;       // |velocity|, recomputed whenever velocity changes
+    double E, w, tc, nmfp;
+    uint64_t seed;
+    int ci, cj, ck, group;
+};
+
+// updateTrajectory (CollisionEvent.cc:25-48)
+__device__ __forceinline__ void update_trajectory(Lane& L, double energy, double angle) {
+    L.E = energy;
+    double cosTheta = angle;
+    double r = rng_sample(L.seed);
+    double phi = 2 * 3.14159265 * r;
+    double sinPhi = sin(phi);
+    double cosPhi = cos(phi);
+    double sinTheta = sqrt((1.0 - (cosTheta * cosTheta)));
+    rotate3d(L.a, L.b, L.g, sinTheta, cosTheta, sinPhi, cosPhi);
+    double speed = (kLight * sqrt((1.0 - ((kRestMass * kRestMass) /
+                                           ((energy + kRestMass) * (energy + kRestMass))))));
+    L.vx = speed * L.a;
+    L.vy = speed * L.b;
+    L.vz = speed * L.g;
+    r = rng_sample(L.seed);
+    L.nmfp = -1.0 * log(r);
+}
+
+__device__ __forceinline__ double vlength(double x, double y, double z) {
+    return sqrt(x * x + y * y + z * z);  // MC_Vector::Length
+}
+
+__device__ __forceinline__ void store_particle(const Particles& o, uint32_t i, const Lane& L, int cell) {
+    o.x[i] = L.x;
+    o.y[i] = L.y;
+    o.z[i] = L.z;
+    o.vx[i] = L.vx;
+    o.vy[i] = L.vy;
+    o.vz[i] = L.vz;
+    o.E[i] = L.E;
+    o.w[i] = L.w;
+    o.tc[i] = L.tc;
+    o.nmfp[i] = L.nmfp;
+    o.seed[i] = L.seed;
+    o.cell[i] = cell;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Reserve `count` (0..4) consecutive slots per active lane with one atomic
+// per wave; returns this lane's first slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t count) {
+    uint64_t act = __ballot(1);
+    uint64_t b0 = __ballot(count > 0), b1 = __ballot(count > 1), b2 = __ballot(count > 2),
+             b3 = __ballot(count > 3);
+    uint32_t total = __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
+    uint64_t lt = lanemask_lt();
+    uint32_t prefix = __popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt) + __popcll(b3 & lt);
+    int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    return base + prefix;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Tables an event needs, staged in LDS when they fit.
+struct Tables {
+    const double* energies;
+    const double* total;
+    const double* macro;
+    const int32_t* reactType;
+    const MatInfo* mat;
+};
+
+template <bool kLds>
+__device__ __forceinline__ Tables stage_tables(const DevProblem& P, double* lds) {
+    Tables t;
+    if (!kLds) {
+        t.energies = P.energies;
+        t.total = P.total;
+        t.macro = P.macro;
+        t.reactType = P.reactType;
+        t.mat = P.mat;  // not reached: the global variant copies MatInfo to LDS too
+        return t;
+    }
+    const int nE = P.G + 1, nT = P.nMat * P.G, nM = P.nMat * P.maxR * P.G;
+    double* sE = lds;
+    double* sT = sE + nE;
+    double* sM = sT + nT;
+    for (int q = threadIdx.x; q < nE; q += blockDim.x) sE[q] = P.energies[q];
+    for (int q = threadIdx.x; q < nT; q += blockDim.x) sT[q] = P.total[q];
+    for (int q = threadIdx.x; q < nM; q += blockDim.x) sM[q] = P.macro[q];
+    t.energies = sE;
+    t.total = sT;
+    t.macro = sM;
+    t.reactType = P.reactType;
+    t.mat = P.mat;
+    return t;
+}
+
+// ------------------------------------------------------------- kernels ----
+
+template <bool kLds>
+__global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, Particles in,
+                                               const uint32_t* __restrict__ nInPtr, Particles census,
+                                               uint32_t censusCap, Particles extra, uint32_t* nExtraOut,
+                                               uint32_t extraCap, PassCounters* pc, DevCounters* dc) {
+    extern __shared__ double lds[];
+    __shared__ MatInfo sMat[kMaxMaterials];
+    __shared__ int32_t sReact[kMaxMaterials * kMaxReactions];
+    Tables T = stage_tables<kLds>(P, lds);
+    for (int q = threadIdx.x; q < P.nMat; q += blockDim.x) sMat[q] = P.mat[q];
+    for (int q = threadIdx.x; q < P.nMat * P.maxR; q += blockDim.x) sReact[q] = P.reactType[q];
+    __syncthreads();
+
+    const uint32_t nIn = *nInPtr;
+    const int G = P.G;
+    const int lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+
+    uint32_t qBase = 0, qEnd = 0;
+    bool exhausted = false;
+    bool active = false;
+    Lane L = {};
+    uint32_t histSeg = 0;
+    uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
+
+    for (;;) {
+        // ---- refill idle lanes from the pass queue (wave-uniform control) ----
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && idle != 0 && (__popcll(idle) >= (uint32_t)tune.refillMin || idle == ~0ull)) {
+            while (idle != 0 && !exhausted) {
+                if (qBase >= qEnd) {
+                    uint32_t got = 0;
+                    if (lane == 0) got = atomicAdd(&pc->head, (uint32_t)tune.chunk);
+                    got = __shfl(got, 0);
+                    if (got >= nIn) {
+                        exhausted = true;
+                        break;
+                    }
+                    qBase = got;
+                    qEnd = min(got + (uint32_t)tune.chunk, nIn);
+                }
+                uint32_t avail = qEnd - qBase;
+                uint32_t nIdle = __popcll(idle);
+                uint32_t rank = __popcll(idle & lt);
+                if (!active && rank < avail) {
+                    // MC_Load_Particle (MC_Load_Particle.cc:11-29) +
+                    // MC_Particle(const MC_Base_Particle&) (MC_Base_Particle.hh:281-320)
+                    uint32_t idx = qBase + rank;
+                    L.x = in.x[idx];
+                    L.y = in.y[idx];
+                    L.z = in.z[idx];
+                    L.vx = in.vx[idx];
+                    L.vy = in.vy[idx];
+                    L.vz = in.vz[idx];
+                    L.E = in.E[idx];
+                    L.w = in.w[idx];
+                    L.tc = in.tc[idx];
+                    L.nmfp = in.nmfp[idx];
+                    L.seed = in.seed[idx];
+                    int cell = in.cell[idx];
+                    L.ci = cell % P.nx;
+                    L.cj = (cell / P.nx) % P.ny;
+                    L.ck = cell / (P.nx * P.ny);
+                    L.speed = vlength(L.vx, L.vy, L.vz);
+                    if (L.speed > 0) {
+                        double factor = 1.0 / L.speed;
+                        L.a = factor * L.vx;
+                        L.b = factor * L.vy;
+                        L.g = factor * L.vz;
+                    } else {
+                        L.a = L.b = L.g = 0.0;
+                    }
+                    if (L.tc <= 0.0) L.tc += P.dt;
+                    L.group = energy_group(T.energies, G, L.E);
+                    histSeg = 0;
+                    active = true;
+                }
+                qBase += min(avail, nIdle);
+                idle = __ballot(!active);
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+
+        // ---- one segment: MC_Segment_Outcome (MC_Segment_Outcome.cc:31-226) ----
+        const int cell = L.ci + P.nx * (L.cj + P.ny * L.ck);
+        const int mat = P.cellMat[cell];
+        bool force = false;
+        if (L.nmfp < 0.0) {
+            force = true;
+            L.nmfp = kSmall;
+        }
+        const double sigt = T.total[mat * G + L.group];
+        const double mfp = (sigt == 0.0) ? kHuge : 1.0 / sigt;
+        if (L.nmfp == 0.0) {
+            double r = rng_sample(L.seed);
+            L.nmfp = -1.0 * log(r);
+        }
+        double dColl = force ? kSmall : L.nmfp * mfp;
+        double dCen = L.speed * L.tc;
+        const Cell9 c9 = cell_nodes(P, L.ci, L.cj, L.ck);
+        const double* pl = P.planes + (size_t)cell * 96;
+        Nearest nf = nearest_facet(P, pl, cell, c9, L.x, L.y, L.z, L.a, L.b, L.g);
+        double dFacet = nf.d;
+        if (force) {
+            dFacet = kHuge;
+            dCen = kHuge;
+            dColl = kTiny;
+        }
+        // MC_Find_Min: strict <, ties to the lower index
+        int outcome = 0;
+        double seg = dColl;
+        if (dFacet < seg) {
+            outcome = 1;
+            seg = dFacet;
+        }
+        if (dCen < seg) {
+            outcome = 2;
+            seg = dCen;
+        }
+        L.nmfp -= seg / mfp;
+        if (outcome == 0) L.nmfp = 0.0;
+        else if (outcome == 2) L.tc = (L.tc < 0.0) ? L.tc : 0.0;
+        if (force) L.nmfp = 0.0;
+        if (seg != 0.0) {
+            L.x += (L.a * seg);
+            L.y += (L.b * seg);
+            L.z += (L.g * seg);
+            double t = seg / L.speed;
+            L.tc -= t;
+            if (L.tc < 0.0) L.tc = 0.0;
+            unsafeAtomicAdd(&P.flux[(size_t)cell * G + L.group], seg * L.w);
+        }
+        cSeg++;
+        histSeg++;
+
+        if (outcome == 0) {
+            // ---- CollisionEvent (CollisionEvent.cc:50-148) ----
+            const MatInfo mi = sMat[mat];
+            double r = rng_sample(L.seed);
+            double cur = sigt * r;
+            int selR = -1;
+            for (int iso = 0; iso < mi.nIso && cur >= 0; iso++) {
+                for (int rr = 0; rr < mi.nReact; rr++) {
+                    cur -= T.macro[(mat * P.maxR + rr) * G + L.group];
+                    if (cur < 0) {
+                        selR = rr;
+                        break;
+                    }
+                }
+            }
+            // The reference leaves the reaction undefined if rounding keeps
+            // cur >= 0 after the full scan; the last reaction is used here.
+            if (selR < 0) selR = mi.nReact - 1;
+            const int type = sReact[mat * P.maxR + selR];
+            // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88)
+            double e0 = 0, e1 = 0, e2 = 0, e3 = 0, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            int nOut = 0;
+            if (type == kScatter) {
+                nOut = 1;
+                r = rng_sample(L.seed);
+                e0 = L.E * (1.0 - (r * (1.0 / mi.mass)));
+                r = rng_sample(L.seed) * 2.0 - 1.0;
+                a0 = r;
+            } else if (type == kFission) {
+                int n = (int)(mi.nuBar + rng_sample(L.seed));
+                nOut = n;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (k < n) {
+                        double rr = rng_sample(L.seed) / 2.0 + 0.5;
+                        double ek = (20 * rr * rr);
+                        rr = rng_sample(L.seed) * 2.0 - 1.0;
+                        if (k == 0) { e0 = ek; a0 = rr; }
+                        if (k == 1) { e1 = ek; a1 = rr; }
+                        if (k == 2) { e2 = ek; a2 = rr; }
+                        if (k == 3) { e3 = ek; a3 = rr; }
+                    }
+                }
+                if (nOut > 4) {
+                    atomicOr(&pc->error, 16u);  // MAX_PRODUCTION_SIZE exceeded
+                    nOut = 4;
+                }
+            }
+            cCol++;
+            if (type == kScatter) cScat++;
+            else if (type == kAbsorption) cAbs++;
+            else if (type == kFission) {
+                cFis++;
+                cProd += (uint32_t)nOut;
+            }
+            if (nOut == 0) {
+                active = false;
+            } else {
+                uint32_t slot = 0;
+                if (nOut > 1) slot = wave_append(nExtraOut, (uint32_t)nOut);
+                // secondaries 1..nOut-1: copies of the pre-collision parent
+                // with spawned seeds (CollisionEvent.cc:125-133)
+#pragma unroll
+                for (int k = 1; k < 4; k++) {
+                    if (k < nOut) {
+                        Lane S = L;
+                        S.seed = rng_spawn(L.seed);
+                        double ek = k == 1 ? e1 : (k == 2 ? e2 : e3);
+                        double ak = k == 1 ? a1 : (k == 2 ? a2 : a3);
+                        update_trajectory(S, ek, ak);
+                        uint32_t o = slot + (uint32_t)(k - 1);
+                        if (o < extraCap) store_particle(extra, o, S, cell);
+                        else atomicOr(&pc->error, 2u);
+                    }
+                }
+                update_trajectory(L, e0, a0);
+                L.speed = vlength(L.vx, L.vy, L.vz);
+                if (nOut > 1) {
+                    uint32_t o = slot + (uint32_t)(nOut - 1);
+                    if (o < extraCap) store_particle(extra, o, L, cell);
+                    else atomicOr(&pc->error, 2u);
+                    active = false;
+                } else {
+                    L.group = energy_group(T.energies, G, L.E);
+                }
+            }
+        } else if (outcome == 1) {
+            // ---- MC_Facet_Crossing_Event (MC_Facet_Crossing_Event.cc:25-70) ----
+            const int face = nf.f >> 2;
+            int ni = L.ci, nj = L.cj, nk = L.ck;
+            if (face == 0) ni++;
+            else if (face == 1) ni--;
+            else if (face == 2) nj++;
+            else if (face == 3) nj--;
+            else if (face == 4) nk++;
+            else nk--;
+            const bool outside = ni < 0 || ni >= P.nx || nj < 0 || nj >= P.ny || nk < 0 || nk >= P.nz;
+            if (!outside) {
+                L.ci = ni;
+                L.cj = nj;
+                L.ck = nk;
+            } else if (P.boundary[face] == kEscape) {
+                cEsc++;
+                active = false;
+            } else {
+                // MCT_Reflect_Particle (MCT.cc:401-429)
+                const double* p = pl + 4 * nf.f;
+                double A = p[0], B = p[1], C = p[2];
+                double dot = 2.0 * (L.a * A + L.b * B + L.g * C);
+                if (dot > 0) {
+                    L.a -= dot * A;
+                    L.b -= dot * B;
+                    L.g -= dot * C;
+                }
+                double speed = L.speed;
+                L.vx = speed * L.a;
+                L.vy = speed * L.b;
+                L.vz = speed * L.g;
+                L.speed = vlength(L.vx, L.vy, L.vz);
+            }
+        } else {
+            // ---- census: processedVault->pushParticle (CycleTracking.cc:104-111) ----
+            uint32_t o = wave_append(&pc->nCensus, 1u);
+            if (o < censusCap) store_particle(census, o, L, cell);
+            else atomicOr(&pc->error, 1u);
+            cCen++;
+            active = false;
+        }
+        if (active && histSeg >= kMaxSegmentsPerHistory) {
+            atomicOr(&pc->error, 8u);
+            active = false;
+        }
+    }
+
+    // ---- wave-reduced balance tallies ----
+    cSeg = wave_sum(cSeg);
+    cCol = wave_sum(cCol);
+    cScat = wave_sum(cScat);
+    cAbs = wave_sum(cAbs);
+    cFis = wave_sum(cFis);
+    cProd = wave_sum(cProd);
+    cEsc = wave_sum(cEsc);
+    cCen = wave_sum(cCen);
+    if (lane == 0) {
+        if (cSeg) atomicAdd(&dc->numSegments, (unsigned long long)cSeg);
+        if (cCol) atomicAdd(&dc->collision, (unsigned long long)cCol);
+        if (cScat) atomicAdd(&dc->scatter, (unsigned long long)cScat);
+        if (cAbs) atomicAdd(&dc->absorb, (unsigned long long)cAbs);
+        if (cFis) atomicAdd(&dc->fission, (unsigned long long)cFis);
+        if (cProd) atomicAdd(&dc->produce, (unsigned long long)cProd);
+        if (cEsc) atomicAdd(&dc->escape, (unsigned long long)cEsc);
+        if (cCen) atomicAdd(&dc->census, (unsigned long long)cCen);
+    }
+}
+
+// MC_SourceNow particle generation (MC_SourceNow.cc:68-151) with
+// MCT_Generate_Coordinate_3D_G (MCT.cc:143-226) and Sample_Isotropic
+// (DirectionCosine.cc:5-13).
+__global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, uint32_t outBase,
+                                                uint64_t nSource, uint32_t cycle) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nSource) return;
+    // cell = last c with srcOffset[c] <= t
+    int lo = 0, hi = P.nCells;  // srcOffset[lo] <= t < srcOffset[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (P.srcOffset[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const int cell = lo;
+    const uint64_t first = P.srcOffset[cell];
+    const uint64_t nCell = P.srcOffset[cell + 1] - first;
+    const uint64_t k = t - first;
+    // cell._sourceTally counts every particle ever sourced in the cell;
+    // the cell id is gid * 2^32 (MC_Domain.cc:390)
+    uint64_t base = (uint64_t)cycle * nCell + k + ((uint64_t)cell << 32);
+    uint64_t seed = rng_spawn(base);
+
+    const int ci = cell % P.nx, cj = (cell / P.nx) % P.ny, ck = cell / (P.nx * P.ny);
+    const Cell9 c9 = cell_nodes(P, ci, cj, ck);
+    const double cx = P.cellCenter[3 * (size_t)cell], cy = P.cellCenter[3 * (size_t)cell + 1],
+                 cz = P.cellCenter[3 * (size_t)cell + 2];
+    double r = rng_sample(seed);
+    double whichVolume = r * 6.0 * P.cellVolume[cell];
+    double currentVolume = 0.0;
+    int facet = -1;
+    Node3 p0 = {0, 0, 0}, p1 = {0, 0, 0}, p2 = {0, 0, 0};
+    while (currentVolume < whichVolume) {
+        facet++;
+        if (facet == 24) break;
+        uint32_t code = c_facetCode[facet];
+        p0 = facet_node(c9, code, 0);
+        p1 = facet_node(c9, code, 1);
+        p2 = facet_node(c9, code, 2);
+        // MCT_Cell_Volume_3D_G_vector_tetDet
+        double v0x = p0.x - cx, v0y = p0.y - cy, v0z = p0.z - cz;
+        double v1x = p1.x - cx, v1y = p1.y - cy, v1z = p1.z - cz;
+        double v2x = p2.x - cx, v2y = p2.y - cy, v2z = p2.z - cz;
+        double sub = v0z * (v1x * v2y - v1y * v2x) + v0y * (v1z * v2x - v1x * v2z) +
+                     v0x * (v1y * v2z - v1z * v2y);
+        currentVolume += sub;
+    }
+    double r1 = rng_sample(seed);
+    double r2 = rng_sample(seed);
+    double r3 = rng_sample(seed);
+    if (r1 + r2 > 1.0) {
+        r1 = 1.0 - r1;
+        r2 = 1.0 - r2;
+    }
+    if (r2 + r3 > 1.0) {
+        double tmp = r3;
+        r3 = 1.0 - r1 - r2;
+        r2 = 1.0 - tmp;
+    } else if (r1 + r2 + r3 > 1.0) {
+        double tmp = r3;
+        r3 = r1 + r2 + r3 - 1.0;
+        r1 = 1.0 - r2 - tmp;
+    }
+    double r4 = 1.0 - r1 - r2 - r3;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (facet >= 0) {
+        x = (r4 * cx + r1 * p0.x + r2 * p1.x + r3 * p2.x);
+        y = (r4 * cy + r1 * p0.y + r2 * p1.y + r3 * p2.y);
+        z = (r4 * cz + r1 * p0.z + r2 * p1.z + r3 * p2.z);
+    }
+    double g = 1.0 - 2.0 * rng_sample(seed);
+    double sineGamma = sqrt((1.0 - (g * g)));
+    double phi = kPi * (2.0 * rng_sample(seed) - 1.0);
+    double a = sineGamma * cos(phi);
+    double b = sineGamma * sin(phi);
+    double E = (P.eMax - P.eMin) * rng_sample(seed) + P.eMin;
+    double speed = kLight * sqrt(E * (E + 2.0 * (kRestMass)) / ((E + kRestMass) * (E + kRestMass)));
+    double nmfp = -1.0 * log(rng_sample(seed));
+    double tc = P.dt * rng_sample(seed);
+    const uint32_t o = outBase + (uint32_t)t;
+    out.x[o] = x;
+    out.y[o] = y;
+    out.z[o] = z;
+    out.vx[o] = speed * a;
+    out.vy[o] = speed * b;
+    out.vz[o] = speed * g;
+    out.E[o] = E;
+    out.w[o] = P.sourceWeight;
+    out.tc[o] = tc;
+    out.nmfp[o] = nmfp;
+    out.seed[o] = seed;
+    out.cell[o] = cell;
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// PopulationControlGuts (PopulationControl.cc:71-122) followed by
+// RouletteLowWeightParticles (PopulationControl.cc:127-171), fused per
+// particle: every decision uses the particle's own RNG stream, so the
+// surviving multiset does not depend on processing order.
+__global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
+                                                    Particles out, uint32_t outCap, int mode,
+                                                    double factor, PassCounters* pc, DevCounters* dc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double cutoff = P.lowWeightCutoff;
+    const double weightCutoff = cutoff * P.sourceWeight;
+    uint64_t seed = 0;
+    double w = 0;
+    int splitFactor = 0;
+    uint32_t kills = 0, splits = 0;
+    bool keep = valid;
+    if (valid) {
+        seed = in.seed[i];
+        w = in.w[i];
+        if (mode == 1) {
+            double r = rng_sample(seed);
+            if (r > factor) {
+                keep = false;
+                kills++;
+            } else {
+                w /= factor;
+            }
+        } else if (mode == 2) {
+            double r = rng_sample(seed);
+            splitFactor = (int)floor(factor);
+            if (r > (factor - splitFactor)) splitFactor--;
+            w /= factor;
+            splits += (uint32_t)splitFactor;
+        }
+    }
+    // outputs: the parent (seed after all spawns) and splitFactor copies
+    // (seed j = hash of the parent seed after j spawns); low-weight roulette
+    // then draws once from each output's own stream.
+    const bool roulette = cutoff > 0.0 && w <= weightCutoff;
+    uint32_t survivors = 0;
+    if (keep) {
+        uint64_t s = seed;
+        for (int j = 0; j < splitFactor; j++) {
+            uint64_t cs = rng_spawn(s);
+            if (roulette) {
+                if (rng_sample(cs) <= cutoff) survivors++;
+                else kills++;
+            } else {
+                survivors++;
+            }
+        }
+        uint64_t ps = s;
+        if (roulette) {
+            if (rng_sample(ps) <= cutoff) survivors++;
+            else kills++;
+        } else {
+            survivors++;
+        }
+    }
+    uint32_t incl = wave_inclusive_scan(survivors);
+    uint32_t base = 0;
+    if (__lane_id() == 63 && incl) base = atomicAdd(&pc->nPopOut, incl);
+    base = __shfl(base, 63);
+    uint32_t o = base + incl - survivors;
+    if (keep && survivors) {
+        const double x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+        const double E = in.E[i], tc = in.tc[i], nmfp = in.nmfp[i];
+        const int32_t cell = in.cell[i];
+        const double wOut = roulette ? w / cutoff : w;
+        uint64_t s = seed;
+        auto emit = [&](uint64_t os) {
+            if (o < outCap) {
+                out.x[o] = x;
+                out.y[o] = y;
+                out.z[o] = z;
+                out.vx[o] = vx;
+                out.vy[o] = vy;
+                out.vz[o] = vz;
+                out.E[o] = E;
+                out.w[o] = wOut;
+                out.tc[o] = tc;
+                out.nmfp[o] = nmfp;
+                out.seed[o] = os;
+                out.cell[o] = cell;
+            } else {
+                atomicOr(&pc->error, 4u);
+            }
+            o++;
+        };
+        for (int j = 0; j < splitFactor; j++) {
+            uint64_t cs = rng_spawn(s);
+            if (roulette) {
+                uint64_t t = cs;
+                if (rng_sample(t) <= cutoff) emit(t);
+            } else {
+                emit(cs);
+            }
+        }
+        if (roulette) {
+            uint64_t t = s;
+            if (rng_sample(t) <= cutoff) emit(t);
+        } else {
+            emit(s);
+        }
+    }
+    kills = wave_sum(kills);
+    splits = wave_sum(splits);
+    if (__lane_id() == 0) {
+        if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
+        if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
+    }
+}
+
+constexpr int kFluxBlock = 256;
+
+__global__ void __launch_bounds__(kFluxBlock) k_flux_partial(double* flux, size_t n, double* partials) {
+    __shared__ double s[kFluxBlock];
+    const size_t per = (n + gridDim.x - 1) / gridDim.x;
+    const size_t lo = (size_t)blockIdx.x * per;
+    const size_t hi = lo + per < n ? lo + per : n;
+    double acc = 0.0;
+    for (size_t q = lo + threadIdx.x; q < hi; q += kFluxBlock) {
+        acc += flux[q];
+        flux[q] = 0.0;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+__global__ void k_flux_final(const double* partials, int n, double* result) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double acc = 0.0;
+        for (int q = 0; q < n; q++) acc += partials[q];
+        *result = acc;
+    }
+}
+
+__global__ void k_clear(uint32_t* p, size_t words) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q < words; q += (size_t)gridDim.x * blockDim.x) p[q] = 0u;
+}
+
+__global__ void k_math(int op, const double* a, const double* b, double* out, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double x = a[q], y = b[q];
+    double r = 0;
+    switch (op) {
+        case 0: r = sqrt(x); break;
+        case 1: r = x / y; break;
+        case 2: r = log(x); break;
+        case 3: r = sin(x); break;
+        case 4: r = cos(x); break;
+        case 5: r = x * y; break;
+        case 6: r = (double)(*(const uint64_t*)&a[q]); break;
+        default: r = 0;
+    }
+    out[q] = r;
+}
+
+bool g_codesReady = false;
+
+void facetNodeCodes(uint32_t* code) {
+    static const int corner[14][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {1, 1, 0, 0},
+                                      {0, 0, 1, 0}, {1, 0, 1, 0}, {0, 1, 1, 0}, {1, 1, 1, 0},
+                                      {1, 0, 0, 1}, {0, 0, 0, 1}, {0, 1, 0, 2}, {0, 0, 0, 2},
+                                      {0, 0, 1, 3}, {0, 0, 0, 3}};
+    static const int tri[24][3] = {{1, 3, 8},  {3, 7, 8},  {7, 5, 8},  {5, 1, 8},  {0, 4, 9},
+                                   {4, 6, 9},  {6, 2, 9},  {2, 0, 9},  {3, 2, 10}, {2, 6, 10},
+                                   {6, 7, 10}, {7, 3, 10}, {0, 1, 11}, {1, 5, 11}, {5, 4, 11},
+                                   {4, 0, 11}, {4, 5, 12}, {5, 7, 12}, {7, 6, 12}, {6, 4, 12},
+                                   {0, 2, 13}, {2, 3, 13}, {3, 1, 13}, {1, 0, 13}};
+    for (int f = 0; f < 24; f++) {
+        uint32_t c = 0;
+        for (int m = 0; m < 3; m++) {
+            const int* o = corner[tri[f][m]];
+            uint32_t xc = (o[3] == 2 || o[3] == 3) ? 2u : (uint32_t)o[0];
+            uint32_t yc = (o[3] == 1 || o[3] == 3) ? 2u : (uint32_t)o[1];
+            uint32_t zc = (o[3] == 1 || o[3] == 2) ? 2u : (uint32_t)o[2];
+            c |= (xc | (yc << 2) | (zc << 4)) << (6 * m);
+        }
+        code[f] = c;
+    }
+}
+
+void ensureCodes(hipStream_t s) {
+    if (g_codesReady) return;
+    uint32_t code[24];
+    facetNodeCodes(code);
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_facetCode), code, sizeof(code), 0, hipMemcpyHostToDevice, s);
+    (void)hipStreamSynchronize(s);
+    g_codesReady = true;
+}
+
+}  // namespace
+
+size_t tableLdsBytes(const DevProblem& P) {
+    return sizeof(double) * ((size_t)(P.G + 1) + (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);
+}
+
+void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, uint32_t outBase,
+                  uint64_t nSource, uint32_t cycle) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    ensureCodes(s);
+    if (nSource == 0) return;
+    unsigned blocks = (unsigned)((nSource + 255) / 256);
+    hipLaunchKernelGGL(k_source, dim3(blocks), dim3(256), 0, s, P, out, outBase, nSource, cycle);
+}
+
+void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t n,
+                      Particles out, uint32_t outCap, int mode, double factor, PassCounters* pc,
+                      DevCounters* dc) {
+    if (n == 0) return;
+    hipStream_t s = (hipStream_t)ctx.stream;
+    unsigned blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_population, dim3(blocks), dim3(256), 0, s, P, in, n, out, outCap, mode, factor,
+                       pc, dc);
+}
+
+constexpr size_t kLdsLimit = 64 * 1024;
+
+void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune,
+                     Particles in, const uint32_t* nIn, uint32_t nInHost, Particles census,
+                     uint32_t censusCap, Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
+                     PassCounters* pc, DevCounters* dc) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    ensureCodes(s);
+    const int block = tune.blockSize;
+    const int wavesPerBlock = block / 64;
+    long long persistentBlocks = (long long)ctx.numCUs * tune.wavesPerCU / wavesPerBlock;
+    long long needed = ((long long)nInHost + block - 1) / block;
+    long long blocks = needed < persistentBlocks ? needed : persistentBlocks;
+    if (blocks < 1) blocks = 1;
+    size_t lds = tableLdsBytes(P);
+    if (lds <= kLdsLimit) {
+        hipLaunchKernelGGL(k_track<true>, dim3((unsigned)blocks), dim3(block), lds, s, P, tune, in, nIn,
+                           census, censusCap, extra, nExtraOut, extraCap, pc, dc);
+    } else {
+        hipLaunchKernelGGL(k_track<false>, dim3((unsigned)blocks), dim3(block), 0, s, P, tune, in, nIn,
+                           census, censusCap, extra, nExtraOut, extraCap, pc, dc);
+    }
+}
+
+void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partials, int nPartials,
+                   double* result) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    hipLaunchKernelGGL(k_flux_partial, dim3(nPartials), dim3(kFluxBlock), 0, s, flux, n, partials);
+    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(64), 0, s, partials, nPartials, result);
+}
+
+void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    size_t words = bytes / 4;
+    unsigned blocks = (unsigned)((words + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, s, (uint32_t*)ptr, words);
+}
+
+void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out,
+                     size_t n) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_math, dim3(blocks), dim3(256), 0, s, op, a, b, out, n);
+}
+
+}  // namespace qs

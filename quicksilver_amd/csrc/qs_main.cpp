@@ -1,0 +1,61 @@
+// qs: command-line driver with the reference's surface (same flags and deck
+// grammar, Parameters.cc:80-255) running the per-cycle loop of main.cc:38-85
+// on one GPU through the C ABI.
+//
+// Output: the parameters as a re-runnable deck, one line per cycle in the
+// layout of Tallies::PrintSummary (Tallies.cc:123-144) with the scalar flux
+// printed at full precision (%.17e), and the figure of merit of
+// MC_Fast_Timer.cc:97-104 (segments / tracking seconds).
+#include <cinttypes>
+#include <cstdio>
+#include <iostream>
+#include <stdexcept>
+#include <vector>
+
+#include "qs_params.h"
+#include "qsg.h"
+
+int main(int argc, char** argv) {
+    try {
+        qs::Parameters params = qs::getParameters(argc, argv);
+        qs::printParameters(params, std::cout);
+    } catch (const std::exception& e) {
+        fprintf(stderr, "qs: %s\n", e.what());
+        return 2;
+    }
+    std::cout.flush();
+    qsg_problem p = nullptr;
+    if (qsg_create(argc, argv, 0, &p) != QSG_OK) {
+        fprintf(stderr, "qs: %s\n", qsg_last_error());
+        return 1;
+    }
+    qsg_info info;
+    qsg_get_info(p, &info);
+    printf("%-8s %12s %12s %12s %12s %12s %12s %12s %12s %12s %12s %12s %12s %24s %14s\n", "cycle", "start",
+           "source", "rr", "split", "absorb", "scatter", "fission", "produce", "collisn", "escape", "census",
+           "num_seg", "scalar_flux", "cycleTracking");
+    uint64_t totalSeg = 0;
+    double totalTrack = 0.0;
+    for (int c = 0; c < info.n_steps; c++) {
+        qsg_balance b;
+        double flux = 0, secs = 0;
+        int rc = qsg_run(p, 1, &b, &flux, &secs);
+        if (rc != QSG_OK) {
+            fprintf(stderr, "qs: cycle %d: %s\n", c, qsg_last_error());
+            qsg_destroy(p);
+            return 1;
+        }
+        printf("%8i %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64
+               " %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64 " %12" PRIu64
+               " %24.17e %14e\n",
+               c, b.start, b.source, b.rr, b.split, b.absorb, b.scatter, b.fission, b.produce, b.collision,
+               b.escape, b.census, b.num_segments, flux, secs);
+        fflush(stdout);
+        totalSeg += b.num_segments;
+        totalTrack += secs;
+    }
+    printf("Figure Of Merit  %14.2e [Num Segments / Cycle Tracking Time]  (%" PRIu64 " segments, %.6e s)\n",
+           totalTrack > 0 ? (double)totalSeg / totalTrack : 0.0, totalSeg, totalTrack);
+    qsg_destroy(p);
+    return 0;
+}

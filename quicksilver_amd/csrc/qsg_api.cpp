@@ -1,0 +1,618 @@
+// C ABI (include/qsg.h) over the device path: problem construction, buffer
+// management and the per-cycle orchestration that replaces main.cc's
+// cycleInit / cycleTracking / cycleFinalize for one problem on one GPU.
+//
+// Particle storage per problem (structure of arrays, 92 B / particle):
+//   proc[2]   processing population of a cycle / census output of the cycle
+//             (roles swap at the end of each cycle, like the reference's
+//             processing/processed vaults, ParticleVaultContainer.cc:278)
+//   pop       population-control output = input of the first tracking pass
+//   extra[2]  secondaries of tracking pass k (ping-pong), input of pass k+1
+//             (the reference's extra vaults, ParticleVaultContainer.cc:341-400)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qs_device.h"
+#include "qs_params.h"
+#include "qs_problem.h"
+#include "qsg.h"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct QsgError : std::runtime_error {
+    int code;
+    QsgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw QsgError(QSG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct PBuf {
+    void* mem = nullptr;
+    qs::Particles v = {};
+    uint32_t cap = 0;
+};
+
+}  // namespace
+
+struct qsg_problem_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int numCUs = 0;
+    qs::Parameters params;
+    qs::ProblemTables tables;
+    qs::DevProblem dp = {};
+    std::vector<void*> allocs;
+    size_t deviceBytes = 0;
+
+    PBuf proc[2], pop, extra[2];
+    int cur = 0;               // proc[cur] holds this cycle's processing population
+    uint64_t startCount = 0;   // census particles carried into this cycle
+    uint32_t nPop = 0;         // processing population after population control
+    uint32_t nCensus = 0;      // census count after tracking
+    int censusBuf = -1;        // proc index holding the last census
+    int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
+    int cycle = 0;
+
+    qs::PassCounters* pc = nullptr;
+    qs::DevCounters* dc = nullptr;
+    qs::PassCounters* hpc = nullptr;  // pinned host mirrors
+    qs::DevCounters* hdc = nullptr;
+    double* partials = nullptr;
+    int nPartials = 0;
+    double* fluxSumDev = nullptr;
+    double* hFluxSum = nullptr;
+
+    qs::TrackTuning tune = {256, 16, 16, 64};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double lastTrackingSeconds = 0.0;
+
+    qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
+
+    template <typename T>
+    T* dalloc(size_t count) {
+        void* p = nullptr;
+        size_t bytes = count * sizeof(T);
+        if (bytes == 0) bytes = 8;
+        hipCheck(hipMalloc(&p, bytes), "hipMalloc");
+        allocs.push_back(p);
+        deviceBytes += bytes;
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& h) {
+        T* d = dalloc<T>(h.size());
+        if (!h.empty())
+            hipCheck(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+        return d;
+    }
+};
+
+namespace {
+
+// Carve the 12 SoA arrays out of one allocation.
+PBuf makeParticles(qsg_problem_s* p, uint32_t cap) {
+    PBuf b;
+    b.cap = cap;
+    const size_t n = cap ? cap : 1;
+    auto padded = [](size_t bytes) { return ((bytes + 255) / 256) * 256; };
+    size_t total = 10 * padded(n * 8) + padded(n * 8) + padded(n * 4);
+    char* m = p->dalloc<char>(total);
+    b.mem = m;
+    size_t off = 0;
+    auto take = [&](size_t elem) {
+        char* q = m + off;
+        off += padded(n * elem);
+        return q;
+    };
+    b.v.x = (double*)take(8);
+    b.v.y = (double*)take(8);
+    b.v.z = (double*)take(8);
+    b.v.vx = (double*)take(8);
+    b.v.vy = (double*)take(8);
+    b.v.vz = (double*)take(8);
+    b.v.E = (double*)take(8);
+    b.v.w = (double*)take(8);
+    b.v.tc = (double*)take(8);
+    b.v.nmfp = (double*)take(8);
+    b.v.seed = (uint64_t*)take(8);
+    b.v.cell = (int32_t*)take(4);
+    return b;
+}
+
+void copyParticles(const qs::Particles& d, const qsg_particles* h, uint64_t n, hipMemcpyKind kind,
+                   hipStream_t s) {
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hipCheck(hipMemcpyAsync(dst, src, bytes, kind, s), "hipMemcpyAsync");
+    };
+    const bool up = kind == hipMemcpyHostToDevice;
+    const size_t n8 = (size_t)n * 8, n4 = (size_t)n * 4;
+    if (up) {
+        cp(d.x, h->x, n8); cp(d.y, h->y, n8); cp(d.z, h->z, n8);
+        cp(d.vx, h->vx, n8); cp(d.vy, h->vy, n8); cp(d.vz, h->vz, n8);
+        cp(d.E, h->energy, n8); cp(d.w, h->weight, n8); cp(d.tc, h->time_to_census, n8);
+        cp(d.nmfp, h->num_mean_free_paths, n8); cp(d.seed, h->seed, n8); cp(d.cell, h->cell, n4);
+    } else {
+        cp(h->x, d.x, n8); cp(h->y, d.y, n8); cp(h->z, d.z, n8);
+        cp(h->vx, d.vx, n8); cp(h->vy, d.vy, n8); cp(h->vz, d.vz, n8);
+        cp(h->energy, d.E, n8); cp(h->weight, d.w, n8); cp(h->time_to_census, d.tc, n8);
+        cp(h->num_mean_free_paths, d.nmfp, n8); cp(h->seed, d.seed, n8); cp(h->cell, d.cell, n4);
+    }
+    hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+void setDevice(qsg_problem_s* p) { hipCheck(hipSetDevice(p->device), "hipSetDevice"); }
+
+void syncCounters(qsg_problem_s* p) {
+    hipCheck(hipMemcpyAsync(p->hpc, p->pc, sizeof(qs::PassCounters), hipMemcpyDeviceToHost, p->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(p->hdc, p->dc, sizeof(qs::DevCounters), hipMemcpyDeviceToHost, p->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+}
+
+void checkErrors(qsg_problem_s* p) {
+    uint32_t e = p->hpc->error;
+    if (!e) return;
+    std::string m = "device error flags:";
+    if (e & 1u) m += " census buffer overflow;";
+    if (e & 2u) m += " secondary buffer overflow;";
+    if (e & 4u) m += " population-control buffer overflow;";
+    if (e & 8u) m += " runaway particle history;";
+    if (e & 16u) m += " fission produced more than 4 particles;";
+    throw QsgError((e & 7u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC,
+                   m + " (raise QSG_CAPACITY_FACTOR for capacity errors)");
+}
+
+void build(qsg_problem_s* p, int device) {
+    p->device = device;
+    setDevice(p);
+    hipCheck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipDeviceGetAttribute(&p->numCUs, hipDeviceAttributeMultiprocessorCount, device),
+             "hipDeviceGetAttribute");
+    hipCheck(hipEventCreate(&p->ev0), "hipEventCreate");
+    hipCheck(hipEventCreate(&p->ev1), "hipEventCreate");
+
+    const qs::ProblemTables& T = p->tables;
+    if ((int)T.materials.size() > qs::kMaxMaterials)
+        throw QsgError(QSG_ERR_ARG, "too many materials for the device tables");
+    if (T.maxReactions > qs::kMaxReactions)
+        throw QsgError(QSG_ERR_ARG, "too many reactions per material for the device tables");
+
+    qs::DevProblem& d = p->dp;
+    d.nx = T.nx;
+    d.ny = T.ny;
+    d.nz = T.nz;
+    d.nCells = T.nCells;
+    d.dx = T.dx;
+    d.dy = T.dy;
+    d.dz = T.dz;
+    d.hdx = T.dx / 2.0;
+    d.hdy = T.dy / 2.0;
+    d.hdz = T.dz / 2.0;
+    for (int f = 0; f < 6; f++) d.boundary[f] = T.boundary[f];
+    d.planes = p->upload(T.planes);
+    d.cellMat = p->upload(T.cellMaterial);
+    d.cellVolume = p->upload(T.cellVolume);
+    d.cellCenter = p->upload(T.cellCenter);
+    d.srcOffset = p->upload(T.cellSourceOffset);
+    d.G = T.nGroups;
+    d.nMat = (int)T.materials.size();
+    d.maxR = T.maxReactions;
+    d.energies = p->upload(T.energies);
+    std::vector<double> total((size_t)d.nMat * d.G), macro((size_t)d.nMat * d.maxR * d.G, 0.0);
+    std::vector<int32_t> react((size_t)d.nMat * d.maxR, 0);
+    for (int m = 0; m < d.nMat; m++) {
+        const qs::MaterialTables& mt = T.materials[m];
+        for (int g = 0; g < d.G; g++) total[(size_t)m * d.G + g] = mt.total[g];
+        for (int r = 0; r < mt.nReactions; r++) {
+            react[(size_t)m * d.maxR + r] = mt.reactionType[r];
+            for (int g = 0; g < d.G; g++)
+                macro[((size_t)m * d.maxR + r) * d.G + g] = mt.macro[(size_t)r * d.G + g];
+        }
+        d.mat[m].nIso = mt.nIsotopes;
+        d.mat[m].nReact = mt.nReactions;
+        d.mat[m].mass = mt.mass;
+        d.mat[m].nuBar = mt.nuBar;
+    }
+    d.total = p->upload(total);
+    d.macro = p->upload(macro);
+    d.reactType = p->upload(react);
+    d.dt = T.dt;
+    d.eMin = T.eMin;
+    d.eMax = T.eMax;
+    d.sourceWeight = T.sourceWeight;
+    d.lowWeightCutoff = T.lowWeightCutoff;
+    d.flux = p->dalloc<double>((size_t)T.nCells * T.nGroups);
+    hipCheck(hipMemset(d.flux, 0, (size_t)T.nCells * T.nGroups * sizeof(double)), "hipMemset");
+
+    // particle buffers
+    double factor = 2.0;
+    if (const char* env = getenv("QSG_CAPACITY_FACTOR")) factor = atof(env);
+    if (!(factor >= 1.0)) factor = 2.0;
+    double capD = factor * (double)T.nParticles + (double)T.sourcePerCycle + 65536.0;
+    if (capD > 4.0e9) throw QsgError(QSG_ERR_CAPACITY, "particle capacity exceeds 32-bit indexing");
+    uint32_t cap = (uint32_t)capD;
+    p->proc[0] = makeParticles(p, cap);
+    p->proc[1] = makeParticles(p, cap);
+    p->pop = makeParticles(p, cap);
+    p->extra[0] = makeParticles(p, cap);
+    p->extra[1] = makeParticles(p, cap);
+
+    p->pc = p->dalloc<qs::PassCounters>(1);
+    p->dc = p->dalloc<qs::DevCounters>(1);
+    hipCheck(hipMemset(p->pc, 0, sizeof(qs::PassCounters)), "hipMemset");
+    hipCheck(hipMemset(p->dc, 0, sizeof(qs::DevCounters)), "hipMemset");
+    hipCheck(hipHostMalloc((void**)&p->hpc, sizeof(qs::PassCounters), hipHostMallocDefault), "hipHostMalloc");
+    hipCheck(hipHostMalloc((void**)&p->hdc, sizeof(qs::DevCounters), hipHostMallocDefault), "hipHostMalloc");
+    hipCheck(hipHostMalloc((void**)&p->hFluxSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    p->nPartials = 1024;
+    p->partials = p->dalloc<double>(p->nPartials);
+    p->fluxSumDev = p->dalloc<double>(1);
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void destroyProblem(qsg_problem_s* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (void* a : p->allocs) (void)hipFree(a);
+    if (p->hpc) (void)hipHostFree(p->hpc);
+    if (p->hdc) (void)hipHostFree(p->hdc);
+    if (p->hFluxSum) (void)hipHostFree(p->hFluxSum);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+// cycleInit (main.cc:96-121): source, population control, low-weight roulette.
+void cycleInit(qsg_problem_s* p) {
+    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_init called twice without finalize");
+    setDevice(p);
+    const qs::ProblemTables& T = p->tables;
+    qs::LaunchCtx c = p->ctx();
+    PBuf& proc = p->proc[p->cur];
+    uint64_t n = p->startCount + T.sourcePerCycle;
+    if (n > proc.cap) throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
+    qs::launchClear(c, p->dc, sizeof(qs::DevCounters));
+    qs::launchClear(c, p->pc, sizeof(qs::PassCounters));
+    qs::launchSource(c, p->dp, proc.v, (uint32_t)p->startCount, T.sourcePerCycle, (uint32_t)p->cycle);
+
+    // PopulationControl (PopulationControl.cc:20-65) on one rank
+    double target = (double)T.nParticles;
+    double factor;
+    if (T.loadBalance) factor = (n != 0) ? target / (double)n : 1.0;
+    else factor = target / (double)n;
+    int mode = 0;
+    if (factor != 1.0 && n != 0) mode = (factor < 1) ? 1 : ((factor > 1) ? 2 : 0);
+    qs::launchPopulation(c, p->dp, proc.v, (uint32_t)n, p->pop.v, p->pop.cap, mode, factor, p->pc, p->dc);
+    syncCounters(p);
+    checkErrors(p);
+    p->nPop = p->hpc->nPopOut;
+    p->phase = 1;
+}
+
+// cycleTracking (main.cc:138-307): passes over the population until no
+// secondaries remain.
+void cycleTracking(qsg_problem_s* p) {
+    if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "cycle_tracking called before cycle_init");
+    setDevice(p);
+    qs::LaunchCtx c = p->ctx();
+    const int out = 1 - p->cur;  // census goes to the other processing buffer
+    PBuf& census = p->proc[out];
+    hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
+    const uint32_t* nIn = &p->pc->nPopOut;
+    uint32_t nInHost = p->nPop;
+    qs::Particles in = p->pop.v;
+    int k = 0;
+    while (nInHost > 0) {
+        uint32_t* nOut = &p->pc->extraCount[k & 1];
+        qs::launchClear(c, nOut, sizeof(uint32_t));
+        qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
+        PBuf& ex = p->extra[k & 1];
+        qs::launchTrackPass(c, p->dp, p->tune, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap,
+                            p->pc, p->dc);
+        hipCheck(hipGetLastError(), "track pass launch");
+        uint32_t next = 0;
+        hipCheck(hipMemcpyAsync(&p->hpc->extraCount[k & 1], nOut, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                p->stream),
+                 "hipMemcpyAsync");
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        next = p->hpc->extraCount[k & 1];
+        if (next > ex.cap) {
+            syncCounters(p);
+            checkErrors(p);
+            throw QsgError(QSG_ERR_CAPACITY, "secondary buffer overflow");
+        }
+        in = ex.v;
+        nIn = nOut;
+        nInHost = next;
+        k++;
+    }
+    hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
+    syncCounters(p);
+    checkErrors(p);
+    float ms = 0;
+    hipCheck(hipEventElapsedTime(&ms, p->ev0, p->ev1), "hipEventElapsedTime");
+    p->lastTrackingSeconds = ms * 1e-3;
+    p->nCensus = p->hpc->nCensus;
+    p->censusBuf = out;
+    p->phase = 2;
+}
+
+// cycleFinalize (main.cc:310-324) + Tallies::CycleFinalize (Tallies.cc:25-98)
+void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
+    if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
+    setDevice(p);
+    qs::LaunchCtx c = p->ctx();
+    const qs::ProblemTables& T = p->tables;
+    qs::launchFluxSum(c, p->dp.flux, (size_t)T.nCells * T.nGroups, p->partials, p->nPartials, p->fluxSumDev);
+    hipCheck(hipMemcpyAsync(p->hFluxSum, p->fluxSumDev, sizeof(double), hipMemcpyDeviceToHost, p->stream),
+             "hipMemcpyAsync");
+    syncCounters(p);
+    checkErrors(p);
+    const qs::DevCounters& d = *p->hdc;
+    if (bal) {
+        bal->start = p->startCount;
+        bal->source = T.sourcePerCycle;
+        bal->rr = d.rr;
+        bal->split = d.split;
+        bal->absorb = d.absorb;
+        bal->scatter = d.scatter;
+        bal->fission = d.fission;
+        bal->produce = d.produce;
+        bal->collision = d.collision;
+        bal->escape = d.escape;
+        bal->census = d.census;
+        bal->num_segments = d.numSegments;
+        bal->end = p->nCensus;
+    }
+    if (fluxSum) *fluxSum = *p->hFluxSum;
+    p->startCount = p->nCensus;
+    p->cur = p->censusBuf;
+    p->cycle++;
+    p->phase = 0;
+}
+
+int fail(const std::exception& e) {
+    g_lastError = e.what();
+    if (const QsgError* q = dynamic_cast<const QsgError*>(&e)) return q->code;
+    return QSG_ERR_ARG;
+}
+
+qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
+    qsg_problem_s* p = new qsg_problem_s();
+    try {
+        p->params = params;
+        p->tables = qs::buildProblem(params);
+        build(p, device);
+    } catch (...) {
+        destroyProblem(p);
+        throw;
+    }
+    return p;
+}
+
+#define QSG_GUARD(...)                   \
+    try {                                \
+        __VA_ARGS__;                     \
+        return QSG_OK;                   \
+    } catch (const std::exception& e) {  \
+        return fail(e);                  \
+    } catch (...) {                      \
+        g_lastError = "unknown error";   \
+        return QSG_ERR_ARG;              \
+    }
+
+}  // namespace
+
+extern "C" {
+
+const char* qsg_last_error(void) { return g_lastError.c_str(); }
+
+int qsg_create(int argc, const char* const* argv, int device, qsg_problem* out) {
+    QSG_GUARD({
+        if (!out) throw QsgError(QSG_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        qs::Parameters params = qs::getParameters(argc, argv);
+        *out = createFromParams(params, device);
+    })
+}
+
+int qsg_create_from_deck(const char* deck_text, int argc, const char* const* argv, int device,
+                         qsg_problem* out) {
+    QSG_GUARD({
+        if (!out || !deck_text) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *out = nullptr;
+        const char* dflt[] = {"qs"};
+        qs::Parameters params = (argc > 0 && argv) ? qs::getParameters(argc, argv) : qs::getParameters(1, dflt);
+        params.geometryParams.clear();
+        // getParameters supplied the default problem; a deck replaces it
+        params.materialParams.clear();
+        params.crossSectionParams.clear();
+        qs::parseDeckText(deck_text, params);
+        qs::supplyDefaults(params);
+        *out = createFromParams(params, device);
+    })
+}
+
+int qsg_destroy(qsg_problem p) {
+    QSG_GUARD({ destroyProblem(p); })
+}
+
+int qsg_get_info(qsg_problem p, qsg_info* info) {
+    QSG_GUARD({
+        if (!p || !info) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        const qs::ProblemTables& T = p->tables;
+        info->nx = T.nx;
+        info->ny = T.ny;
+        info->nz = T.nz;
+        info->n_groups = T.nGroups;
+        info->n_materials = (int32_t)T.materials.size();
+        info->n_steps = T.nSteps;
+        info->n_cells = (uint64_t)T.nCells;
+        info->n_particles_target = T.nParticles;
+        info->source_per_cycle = T.sourcePerCycle;
+        info->particle_capacity = p->pop.cap;
+        info->device_bytes = p->deviceBytes;
+        info->dt = T.dt;
+        info->source_weight = T.sourceWeight;
+        info->cycle = p->cycle;
+        info->lds_tables = qs::tableLdsBytes(p->dp) <= 64 * 1024 ? 1 : 0;
+    })
+}
+
+int qsg_set_tuning(qsg_problem p, const qsg_tuning* t) {
+    QSG_GUARD({
+        if (!p || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (t->block_size < 64 || t->block_size > 256 || t->block_size % 64)
+            throw QsgError(QSG_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+        if (t->waves_per_cu < 1 || t->waves_per_cu > 64) throw QsgError(QSG_ERR_ARG, "waves_per_cu out of range");
+        if (t->refill_min < 1 || t->refill_min > 64) throw QsgError(QSG_ERR_ARG, "refill_min out of range");
+        if (t->chunk < 1 || t->chunk > 4096) throw QsgError(QSG_ERR_ARG, "chunk out of range");
+        p->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+    })
+}
+
+int qsg_get_tuning(qsg_problem p, qsg_tuning* t) {
+    QSG_GUARD({
+        if (!p || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        t->block_size = p->tune.blockSize;
+        t->waves_per_cu = p->tune.wavesPerCU;
+        t->refill_min = p->tune.refillMin;
+        t->chunk = p->tune.chunk;
+    })
+}
+
+int qsg_cycle_init(qsg_problem p) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        cycleInit(p);
+    })
+}
+
+int qsg_cycle_tracking(qsg_problem p) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        cycleTracking(p);
+    })
+}
+
+int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        cycleFinalize(p, balance, flux_sum);
+    })
+}
+
+int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums, double* tracking_seconds) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        for (int i = 0; i < n_cycles; i++) {
+            cycleInit(p);
+            cycleTracking(p);
+            if (tracking_seconds) tracking_seconds[i] = p->lastTrackingSeconds;
+            cycleFinalize(p, balances ? &balances[i] : nullptr, flux_sums ? &flux_sums[i] : nullptr);
+        }
+    })
+}
+
+int qsg_get_scalar_flux(qsg_problem p, double* out, uint64_t n) {
+    QSG_GUARD({
+        if (!p || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        uint64_t total = (uint64_t)p->tables.nCells * p->tables.nGroups;
+        if (n != total) throw QsgError(QSG_ERR_ARG, "flux buffer must hold n_cells * n_groups values");
+        setDevice(p);
+        hipCheck(hipMemcpyAsync(out, p->dp.flux, n * sizeof(double), hipMemcpyDeviceToHost, p->stream),
+                 "hipMemcpyAsync");
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    })
+}
+
+int qsg_set_particles(qsg_problem p, const qsg_particles* in, uint64_t n) {
+    QSG_GUARD({
+        if (!p || !in) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "set_particles must follow cycle_init");
+        if (n > p->pop.cap) throw QsgError(QSG_ERR_CAPACITY, "too many particles");
+        for (uint64_t i = 0; i < n; i++)
+            if (in->cell[i] < 0 || in->cell[i] >= p->tables.nCells)
+                throw QsgError(QSG_ERR_ARG, "particle cell id out of range");
+        setDevice(p);
+        copyParticles(p->pop.v, in, n, hipMemcpyHostToDevice, p->stream);
+        uint32_t n32 = (uint32_t)n;
+        hipCheck(hipMemcpy(&p->pc->nPopOut, &n32, sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy");
+        p->nPop = n32;
+    })
+}
+
+int qsg_get_census_count(qsg_problem p, uint64_t* n) {
+    QSG_GUARD({
+        if (!p || !n) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *n = (p->censusBuf < 0) ? 0 : p->nCensus;
+    })
+}
+
+int qsg_get_census(qsg_problem p, qsg_particles* out, uint64_t n) {
+    QSG_GUARD({
+        if (!p || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (p->censusBuf < 0) throw QsgError(QSG_ERR_STATE, "no census yet");
+        if (n > p->nCensus) throw QsgError(QSG_ERR_ARG, "n exceeds the census count");
+        setDevice(p);
+        copyParticles(p->proc[p->censusBuf].v, out, n, hipMemcpyDeviceToHost, p->stream);
+    })
+}
+
+int qsg_get_processing_count(qsg_problem p, uint64_t* n) {
+    QSG_GUARD({
+        if (!p || !n) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *n = (p->phase >= 1) ? p->nPop : 0;
+    })
+}
+
+int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n) {
+    QSG_GUARD({
+        if (!p || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "processing population exists after cycle_init");
+        if (n > p->nPop) throw QsgError(QSG_ERR_ARG, "n exceeds the processing count");
+        setDevice(p);
+        copyParticles(p->pop.v, out, n, hipMemcpyDeviceToHost, p->stream);
+    })
+}
+
+int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device) {
+    QSG_GUARD({
+        if (!a || !b || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s;
+        hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        double *da = nullptr, *db = nullptr, *dout = nullptr;
+        size_t bytes = (size_t)(n ? n : 1) * sizeof(double);
+        hipCheck(hipMalloc(&da, bytes), "hipMalloc");
+        hipCheck(hipMalloc(&db, bytes), "hipMalloc");
+        hipCheck(hipMalloc(&dout, bytes), "hipMalloc");
+        hipCheck(hipMemcpy(da, a, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hipCheck(hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        qs::LaunchCtx c{(void*)s, 0};
+        qs::launchMathProbe(c, op, da, db, dout, n);
+        hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+        hipCheck(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        (void)hipFree(da);
+        (void)hipFree(db);
+        (void)hipFree(dout);
+        (void)hipStreamDestroy(s);
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Writes the parity decks and runs the reference CPU path on them.
+
+Fixtures written next to this script:
+  decks/<name>.inp          the input deck (reference deck format)
+  <name>.json               per-cycle balance tallies (bit-exact integers)
+                            and scalar-flux sums (%.17e, as JSON floats)
+  tiny_cube_flux.npy        per-(cell, group) scalar flux of the last cycle
+  tiny_cube_census.npz      census particles of the last cycle
+
+The reference is the unmodified Quicksilver CPU path built by
+oracle/Makefile into oracle/_ref/qs_ref (run from this container, where the
+reference sources exist).  Usage:
+    python tests/golden/make_golden.py [name ...]
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from quicksilver_amd import decks as D  # noqa: E402
+
+QS_REF = os.path.join(ROOT, "oracle", "_ref", "qs_ref")
+FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce",
+          "collision", "escape", "census", "num_segments", "end"]
+
+
+def fixture_decks():
+    return {
+        "small_cube": D.small_cube(),
+        "small_cube_short": D.small_cube(n_particles=20000, n_steps=3),
+        "tiny_cube": D.small_cube(n=5, n_particles=5000, n_steps=3),
+        "nested_boxes": D.nested_boxes(),
+        "nested_small": D.nested_boxes(n=12, n_particles=20000, n_steps=4),
+        "checker_small": D.checkerboard(n=40, blocks=8, n_particles=1000000, n_steps=3),
+        "all_absorb": D.limit_case("all_absorb"),
+        "all_escape": D.limit_case("all_escape"),
+        "no_collisions": D.limit_case("no_collisions", n_particles=20000, n_steps=4),
+        "scatter_only": D.limit_case("scatter_only", n_particles=20000, n_steps=4),
+        "no_fission": D.limit_case("no_fission", n_particles=20000, n_steps=4),
+        "nonflat": D.nonflat(),
+    }
+
+
+def run_reference(deck_path: str, threads: int = 8, env_extra=None):
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    if env_extra:
+        env.update(env_extra)
+    out = subprocess.run([QS_REF, "-i", deck_path], capture_output=True, text=True, env=env,
+                         check=True, cwd=os.path.dirname(deck_path)).stdout
+    cycles, fom = [], None
+    for line in out.splitlines():
+        tok = line.split()
+        if tok and tok[0] == "QSREF":
+            vals = [int(t) for t in tok[2:15]]
+            cycles.append({**dict(zip(FIELDS, vals)), "flux_sum": float(tok[15]),
+                           "tracking_seconds": float(tok[16])})
+        elif tok and tok[0] == "QSREF_FOM":
+            fom = {"segments": int(tok[1]), "tracking_seconds": float(tok[2]),
+                   "segments_per_second": float(tok[3]), "threads": int(tok[4])}
+    return cycles, fom
+
+
+def main(names):
+    os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
+    all_decks = fixture_decks()
+    for name in names or list(all_decks):
+        deck = all_decks[name]
+        path = deck.write(os.path.join(HERE, "decks", name + ".inp"))
+        env = {}
+        tmp = tempfile.mkdtemp()
+        if name == "tiny_cube":
+            env = {"QSREF_FLUX_DUMP": os.path.join(tmp, "flux.bin"),
+                   "QSREF_CENSUS_DUMP": os.path.join(tmp, "census.txt")}
+        cycles, fom = run_reference(path, env_extra=env)
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump({"deck": f"decks/{name}.inp", "source": "oracle/_ref/qs_ref (reference CPU path)",
+                       "cycles": cycles, "fom": fom}, f, indent=1)
+        if name == "tiny_cube":
+            n_cells = deck.n_cells
+            flux = np.fromfile(env["QSREF_FLUX_DUMP"], dtype=np.float64).reshape(-1, n_cells, D.N_GROUPS)
+            np.save(os.path.join(HERE, "tiny_cube_flux.npy"), flux[-1])
+            cols = np.loadtxt(env["QSREF_CENSUS_DUMP"], dtype=str, ndmin=2)
+            np.savez_compressed(os.path.join(HERE, "tiny_cube_census.npz"),
+                                seed=cols[:, 0].astype(np.uint64),
+                                state=cols[:, 1:11].astype(np.float64),
+                                cell=cols[:, 11].astype(np.int32))
+        print(name, len(cycles), "cycles", fom, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,87 @@
+"""Device path vs the reference CPU path on the same decks.
+
+Fixtures: tests/golden/<deck>.json, written by tests/golden/make_golden.py
+from the unmodified reference (oracle/_ref/qs_ref).  Bar: the 13 balance
+tallies of every cycle are bit-exact; the per-cycle scalar-flux sum agrees
+to 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import quicksilver_amd as q
+from conftest import golden_names, load_golden
+
+FLUX_RTOL = 1e-10
+
+DECKS = [n for n in golden_names()]
+
+
+def _run(name):
+    g = load_golden(name)
+    with q.Problem.from_file(g["deck_path"]) as p:
+        out = p.run()
+    return g, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", DECKS)
+def test_balance_and_flux_match_reference(name):
+    print(f"[parity] deck={name}", flush=True)
+    g, out = _run(name)
+    assert len(out) == len(g["cycles"])
+    for c, (ref, (bal, flux, _)) in enumerate(zip(g["cycles"], out)):
+        for f in q.BALANCE_FIELDS:
+            assert bal[f] == ref[f], f"cycle {c} {f}: device {bal[f]} reference {ref[f]}"
+        rf = ref["flux_sum"]
+        if rf == 0.0:
+            assert flux == 0.0
+        else:
+            assert abs(flux - rf) <= FLUX_RTOL * abs(rf), f"cycle {c} flux {flux!r} vs {rf!r}"
+
+
+@pytest.mark.gpu
+def test_tiny_cube_flux_per_bin_and_census_particles():
+    """Per-(cell, group) flux of the last cycle and every census particle
+    (matched by its 64-bit seed) against the reference's dumps."""
+    import os
+    from conftest import GOLDEN
+    g = load_golden("tiny_cube")
+    ref_flux = np.load(os.path.join(GOLDEN, "tiny_cube_flux.npy"))
+    cen = np.load(os.path.join(GOLDEN, "tiny_cube_census.npz"))
+    with q.Problem.from_file(g["deck_path"]) as p:
+        n = p.info().n_steps
+        p.run(n - 1)
+        p.cycle_init()
+        p.cycle_tracking()
+        flux = p.scalar_flux()
+        census = p.census()
+        bal, _ = p.cycle_finalize()
+    assert flux.shape == ref_flux.shape
+    scale = np.abs(ref_flux).max()
+    np.testing.assert_allclose(flux, ref_flux, rtol=1e-10, atol=1e-12 * scale)
+    assert bal["census"] == len(cen["seed"]) == len(census)
+    order_d = np.argsort(census.seed)
+    order_r = np.argsort(cen["seed"])
+    np.testing.assert_array_equal(census.seed[order_d], cen["seed"][order_r])
+    np.testing.assert_array_equal(census.cell[order_d], cen["cell"][order_r])
+    np.testing.assert_allclose(census.state()[order_d], cen["state"][order_r], rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.gpu
+def test_device_fp64_builtins_round_like_ieee():
+    """sqrt, /, * and u64->f64 are correctly rounded on the device (bit-equal
+    to numpy); log/sin/cos stay within 2 ulp of the host libm."""
+    rng = np.random.default_rng(7)
+    a = rng.random(200000) * 10.0 ** rng.integers(-20, 20, 200000)
+    b = rng.random(200000) * 10.0 ** rng.integers(-20, 20, 200000) + 1e-300
+    np.testing.assert_array_equal(q.math_probe(0, a), np.sqrt(a))
+    np.testing.assert_array_equal(q.math_probe(1, a, b), a / b)
+    np.testing.assert_array_equal(q.math_probe(5, a, b), a * b)
+    u = rng.integers(0, 2**63, 200000, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    np.testing.assert_array_equal(q.math_probe(6, u.view(np.float64)), u.astype(np.float64))
+    r = rng.random(200000)
+    for op, fn, x in [(2, np.log, r), (3, np.sin, r * 6.2831853), (4, np.cos, r * 6.2831853)]:
+        d = q.math_probe(op, x)
+        h = fn(x)
+        ulp = np.spacing(np.abs(h))
+        assert np.max(np.abs(d - h) / ulp) <= 2.0

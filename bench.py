@@ -1,0 +1,156 @@
+"""Benchmark of the per-cycle particle-tracking path on one MI355X.
+
+Workload (one "step" = one time step / cycle of the transport problem):
+  nested_boxes deck (tests/golden/decks/nested_boxes.inp): 40^3-cell mesh
+  (24 facets per cell), three materials in nested boxes, 10 isotopes x 3
+  reactions each with seeded polynomial cross sections over 230 groups,
+  1e6 particles, dt = 1e-8 s, FP64.  The first W cycles are warm-up, the next
+  K cycles are timed (the deck's 10 steps for the defaults W=2, K=8).
+
+Printed: one JSON line with
+  value      segments/s over the timed cycles, whole cycle (source +
+             population control + tracking + tally reduction), wall clock,
+             particle state resident in HBM throughout
+  roofline   tracking-pass kernel: algorithmic bytes (per-segment model below)
+             / summed kernel time from HIP events on the library's stream
+  cpu_baseline  the reference CPU path (oracle/_ref/qs_ref, OpenMP) on the
+             first steps of the same deck, timed on this host
+
+Algorithmic bytes per segment (reference data model):
+  B_seg = 8 (sigma_t) + 768 (24 planes x 32 B) + 336 (14 nodes x 24 B)
+        + 144 (~12 facets x 3 node ids x 4 B) + 16 (flux read+write)
+        + 40 f_fc (adjacency) + f_col (16 nIso + 8 nIso nReact) (collision scan)
+        + 272 L/S (136 B particle record loaded + stored per history load)
+  with f_fc, f_col and L/S taken from the run's own bit-exact tallies.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DECK = os.path.join(ROOT, "tests", "golden", "decks", "nested_boxes.inp")
+QS_REF = os.path.join(ROOT, "oracle", "_ref", "qs_ref")
+METRIC = "segments/sec on 1 MI355X, batched vs one-at-a-time; p50/p99 solve latency"
+
+
+def bytes_per_segment(stats: dict, n_iso: int, n_react: int) -> float:
+    S = max(stats["segments"], 1)
+    f_fc = stats["facet_crossings"] / S
+    f_col = stats["collisions"] / S
+    loads = stats["particle_loads"] / S
+    return (8 + 768 + 336 + 144 + 16 + 40 * f_fc + f_col * (16 * n_iso + 8 * n_iso * n_react)
+            + 272 * loads)
+
+
+def cpu_baseline(steps: int):
+    if not os.path.exists(QS_REF):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with open(DECK) as f:
+        text = f.read()
+    text = text.replace("nSteps: 10", f"nSteps: {steps}")
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = os.path.join(tmp, "sample.inp")
+        with open(deck, "w") as f:
+            f.write(text)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        out = subprocess.run([QS_REF, "-i", deck], capture_output=True, text=True, env=env, cwd=tmp,
+                             timeout=600, check=True).stdout
+    for line in out.splitlines():
+        tok = line.split()
+        if tok and tok[0] == "QSREF_FOM":
+            return {"value": float(tok[3]), "unit": "segments/s", "cores": int(tok[4]),
+                    "kind": "reference",
+                    "sample": f"first {steps} of the 10 steps of the same deck, reference OpenMP CPU "
+                              f"path, {tok[1]} segments / {float(tok[2]):.3f} s tracking time "
+                              "(the reference FOM; excludes source and population control)"}
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the problem is "
+              "sharded by particles or by whole problems across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import quicksilver_amd as q
+    with q.Problem.from_file(DECK) as p:
+        if args.warmup:
+            p.run(args.warmup)
+        p.reset_stats()
+        t0 = time.perf_counter()
+        out = p.run(args.steps)
+        t1 = time.perf_counter()
+        stats = p.stats()
+        tune = p.tuning()
+    wall = t1 - t0
+    segments = sum(b["num_segments"] for b, _, _ in out)
+    value = segments / wall
+    bseg = bytes_per_segment(stats, n_iso=10, n_react=3)
+    kernel_s = stats["track_kernel_seconds"]
+    achieved = bseg * stats["segments"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    launches = max(stats["track_launches"], 1)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / max(args.steps, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "nested_boxes: 40^3 cells x 24 facets, 3 materials x 10 isotopes x 3 "
+                        "reactions, 230 groups, polynomial XS, 1e6 particles, dt 1e-8 s",
+            "deck": "tests/golden/decks/nested_boxes.inp",
+            "segments_timed": segments,
+            "mode": "plain (one problem, persistent tracking passes)",
+            "tuning": {"block_size": tune.block_size, "waves_per_cu": tune.waves_per_cu,
+                       "refill_min": tune.refill_min, "chunk": tune.chunk},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_track",
+            "bytes_per_segment": bseg,
+            "segments_per_launch": stats["segments"] / launches,
+            "avg_launch_ms": kernel_s / launches * 1e3,
+        },
+        "tracking_kernel_segments_per_s": stats["segments"] / kernel_s if kernel_s > 0 else None,
+        "phase_seconds": {"init": stats["init_seconds"], "tracking": stats["tracking_seconds"],
+                          "track_kernels": kernel_s, "finalize": stats["finalize_seconds"],
+                          "wall": wall},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

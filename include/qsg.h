@@ -104,6 +104,20 @@ int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum);
 int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums,
             double* tracking_seconds);
 
+/* Device timings accumulated since creation (or the last qsg_reset_stats),
+ * measured with HIP events on the problem's stream. */
+typedef struct qsg_stats {
+    double track_kernel_seconds; /* sum over tracking-pass kernel launches   */
+    double tracking_seconds;     /* cycle tracking phases, first to last pass */
+    double init_seconds;         /* source + population control              */
+    double finalize_seconds;     /* flux reduction + counter readback        */
+    uint64_t track_launches;     /* tracking-pass kernel launches            */
+    uint64_t segments;           /* segments tracked                         */
+    uint64_t collisions, facet_crossings, particle_loads, cycles;
+} qsg_stats;
+int qsg_get_stats(qsg_problem p, qsg_stats* stats);
+int qsg_reset_stats(qsg_problem p);
+
 /* Scalar flux [n_cells][n_groups] accumulated so far in this cycle. */
 int qsg_get_scalar_flux(qsg_problem p, double* out, uint64_t n);
 

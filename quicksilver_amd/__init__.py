@@ -29,7 +29,7 @@ EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy"
            "qsg_set_tuning", "qsg_get_tuning", "qsg_cycle_init", "qsg_cycle_tracking",
            "qsg_cycle_finalize", "qsg_run", "qsg_get_scalar_flux", "qsg_set_particles",
            "qsg_get_census_count", "qsg_get_census", "qsg_get_processing_count",
-           "qsg_get_processing", "qsg_math_probe"]
+           "qsg_get_processing", "qsg_math_probe", "qsg_get_stats", "qsg_reset_stats"]
 
 
 class QsgError(RuntimeError):
@@ -53,6 +53,17 @@ class Info(ctypes.Structure):
                 ("particle_capacity", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("dt", ctypes.c_double), ("source_weight", ctypes.c_double),
                 ("cycle", ctypes.c_int32), ("lds_tables", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("track_kernel_seconds", ctypes.c_double), ("tracking_seconds", ctypes.c_double),
+                ("init_seconds", ctypes.c_double), ("finalize_seconds", ctypes.c_double),
+                ("track_launches", ctypes.c_uint64), ("segments", ctypes.c_uint64),
+                ("collisions", ctypes.c_uint64), ("facet_crossings", ctypes.c_uint64),
+                ("particle_loads", ctypes.c_uint64), ("cycles", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Tuning(ctypes.Structure):
@@ -146,6 +157,8 @@ def lib() -> ctypes.CDLL:
     L.qsg_get_processing_count.argtypes = [vp, _P(ctypes.c_uint64)]
     L.qsg_get_processing.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
     L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
+    L.qsg_get_stats.argtypes = [vp, _P(Stats)]
+    L.qsg_reset_stats.argtypes = [vp]
     _lib = L
     return L
 
@@ -212,6 +225,14 @@ class Problem:
     def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
         t = Tuning(block_size, waves_per_cu, refill_min, chunk)
         _check(lib().qsg_set_tuning(self._h, ctypes.byref(t)))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(lib().qsg_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().qsg_reset_stats(self._h))
 
     # ---- per-cycle phases (main.cc:55-66)
     def cycle_init(self):

@@ -75,8 +75,10 @@ struct qsg_problem_s {
     double* hFluxSum = nullptr;
 
     qs::TrackTuning tune = {256, 16, 16, 64};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evI0 = nullptr, evI1 = nullptr, evF0 = nullptr, evF1 = nullptr;
+    std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
     double lastTrackingSeconds = 0.0;
+    qsg_stats stats = {};
 
     qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
 
@@ -181,8 +183,8 @@ void build(qsg_problem_s* p, int device) {
     hipCheck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
     hipCheck(hipDeviceGetAttribute(&p->numCUs, hipDeviceAttributeMultiprocessorCount, device),
              "hipDeviceGetAttribute");
-    hipCheck(hipEventCreate(&p->ev0), "hipEventCreate");
-    hipCheck(hipEventCreate(&p->ev1), "hipEventCreate");
+    for (hipEvent_t* e : {&p->ev0, &p->ev1, &p->evI0, &p->evI1, &p->evF0, &p->evF1})
+        hipCheck(hipEventCreate(e), "hipEventCreate");
 
     const qs::ProblemTables& T = p->tables;
     if ((int)T.materials.size() > qs::kMaxMaterials)
@@ -271,8 +273,9 @@ void destroyProblem(qsg_problem_s* p) {
     if (p->hpc) (void)hipHostFree(p->hpc);
     if (p->hdc) (void)hipHostFree(p->hdc);
     if (p->hFluxSum) (void)hipHostFree(p->hFluxSum);
-    if (p->ev0) (void)hipEventDestroy(p->ev0);
-    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    for (hipEvent_t e : {p->ev0, p->ev1, p->evI0, p->evI1, p->evF0, p->evF1})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->passEv) (void)hipEventDestroy(e);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -286,6 +289,7 @@ void cycleInit(qsg_problem_s* p) {
     PBuf& proc = p->proc[p->cur];
     uint64_t n = p->startCount + T.sourcePerCycle;
     if (n > proc.cap) throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
+    hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
     qs::launchClear(c, p->dc, sizeof(qs::DevCounters));
     qs::launchClear(c, p->pc, sizeof(qs::PassCounters));
     qs::launchSource(c, p->dp, proc.v, (uint32_t)p->startCount, T.sourcePerCycle, (uint32_t)p->cycle);
@@ -298,9 +302,13 @@ void cycleInit(qsg_problem_s* p) {
     int mode = 0;
     if (factor != 1.0 && n != 0) mode = (factor < 1) ? 1 : ((factor > 1) ? 2 : 0);
     qs::launchPopulation(c, p->dp, proc.v, (uint32_t)n, p->pop.v, p->pop.cap, mode, factor, p->pc, p->dc);
+    hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
     syncCounters(p);
     checkErrors(p);
     p->nPop = p->hpc->nPopOut;
+    float ms = 0;
+    hipCheck(hipEventElapsedTime(&ms, p->evI0, p->evI1), "hipEventElapsedTime");
+    p->stats.init_seconds += ms * 1e-3;
     p->phase = 1;
 }
 
@@ -322,9 +330,16 @@ void cycleTracking(qsg_problem_s* p) {
         qs::launchClear(c, nOut, sizeof(uint32_t));
         qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
         PBuf& ex = p->extra[k & 1];
+        while ((int)p->passEv.size() < 2 * (k + 1)) {
+            hipEvent_t e;
+            hipCheck(hipEventCreate(&e), "hipEventCreate");
+            p->passEv.push_back(e);
+        }
+        hipCheck(hipEventRecord(p->passEv[2 * k], p->stream), "hipEventRecord");
         qs::launchTrackPass(c, p->dp, p->tune, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap,
                             p->pc, p->dc);
         hipCheck(hipGetLastError(), "track pass launch");
+        hipCheck(hipEventRecord(p->passEv[2 * k + 1], p->stream), "hipEventRecord");
         uint32_t next = 0;
         hipCheck(hipMemcpyAsync(&p->hpc->extraCount[k & 1], nOut, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                 p->stream),
@@ -347,6 +362,13 @@ void cycleTracking(qsg_problem_s* p) {
     float ms = 0;
     hipCheck(hipEventElapsedTime(&ms, p->ev0, p->ev1), "hipEventElapsedTime");
     p->lastTrackingSeconds = ms * 1e-3;
+    p->stats.tracking_seconds += ms * 1e-3;
+    for (int q = 0; q < k; q++) {
+        float pm = 0;
+        hipCheck(hipEventElapsedTime(&pm, p->passEv[2 * q], p->passEv[2 * q + 1]), "hipEventElapsedTime");
+        p->stats.track_kernel_seconds += pm * 1e-3;
+    }
+    p->stats.track_launches += (uint64_t)k;
     p->nCensus = p->hpc->nCensus;
     p->censusBuf = out;
     p->phase = 2;
@@ -358,12 +380,27 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
     setDevice(p);
     qs::LaunchCtx c = p->ctx();
     const qs::ProblemTables& T = p->tables;
+    hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
     qs::launchFluxSum(c, p->dp.flux, (size_t)T.nCells * T.nGroups, p->partials, p->nPartials, p->fluxSumDev);
+    hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
     hipCheck(hipMemcpyAsync(p->hFluxSum, p->fluxSumDev, sizeof(double), hipMemcpyDeviceToHost, p->stream),
              "hipMemcpyAsync");
     syncCounters(p);
     checkErrors(p);
     const qs::DevCounters& d = *p->hdc;
+    {
+        float ms = 0;
+        hipCheck(hipEventElapsedTime(&ms, p->evF0, p->evF1), "hipEventElapsedTime");
+        p->stats.finalize_seconds += ms * 1e-3;
+        p->stats.segments += d.numSegments;
+        p->stats.collisions += d.collision;
+        p->stats.facet_crossings += d.numSegments - d.collision - d.census;
+        // particle loads: every population-control survivor plus every
+        // secondary record (start + source + split - rr + produce when each
+        // fission emits >= 2 particles)
+        p->stats.particle_loads += p->startCount + T.sourcePerCycle + d.split - d.rr + d.produce;
+        p->stats.cycles += 1;
+    }
     if (bal) {
         bal->start = p->startCount;
         bal->source = T.sourcePerCycle;
@@ -526,6 +563,20 @@ int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sum
             if (tracking_seconds) tracking_seconds[i] = p->lastTrackingSeconds;
             cycleFinalize(p, balances ? &balances[i] : nullptr, flux_sums ? &flux_sums[i] : nullptr);
         }
+    })
+}
+
+int qsg_get_stats(qsg_problem p, qsg_stats* stats) {
+    QSG_GUARD({
+        if (!p || !stats) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *stats = p->stats;
+    })
+}
+
+int qsg_reset_stats(qsg_problem p) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        p->stats = qsg_stats{};
     })
 }
 

@@ -201,6 +201,23 @@ def nonflat(n_particles: int = 20000, n_steps: int = 4, dt: float = 1e-10) -> De
     return deck
 
 
+def spheres(n_particles: int = 20000, n_steps: int = 4, n: int = 12) -> Deck:
+    """Sphere geometries (Parameters.cc:451-456, MC_Domain.cc:416-422): a
+    reflecting cube of flat material with two overlapping spheres of
+    different materials (the later sphere wins where they overlap)."""
+    sim = _simulation(n, n_particles, n_steps)
+    mats = [_flat_material("background", 0.5, 0.3, 0.02, 0.68, 1e10, n_iso=2, n_react=3),
+            _flat_material("shellA", 2.0, 0.6, 0.02, 0.38, 1e10, n_iso=3, n_react=4),
+            _flat_material("shellB", 1.0, 0.5, 0.04, 0.46, 1e10, n_iso=4, n_react=5)]
+    geo = [_brick("background", (0, 0, 0), (n, n, n)),
+           {"material": "shellA", "shape": "sphere", "radius": n / 3.0,
+            "xCenter": n / 2.0, "yCenter": n / 2.0, "zCenter": n / 2.0},
+           {"material": "shellB", "shape": "sphere", "radius": n / 4.0,
+            "xCenter": n / 3.0, "yCenter": n / 2.0, "zCenter": n / 2.0}]
+    xs = [{"name": "flat", "A": 0.0, "B": 0.0, "C": 0.0, "D": 0.0, "E": 1.0, "nuBar": 2.0}]
+    return Deck(sim, geo, mats, xs, "spheres")
+
+
 # --- seeded polynomial cross sections -------------------------------------
 
 def _group_mid_log10() -> np.ndarray:

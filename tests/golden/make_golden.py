@@ -47,6 +47,7 @@ def fixture_decks():
         "scatter_only": D.limit_case("scatter_only", n_particles=20000, n_steps=4),
         "no_fission": D.limit_case("no_fission", n_particles=20000, n_steps=4),
         "nonflat": D.nonflat(),
+        "spheres": D.spheres(),
     }
 
 

@@ -56,7 +56,7 @@ struct DevProblem {
     int nx, ny, nz, nCells;
     double dx, dy, dz;     // lx/nx ...
     double hdx, hdy, hdz;  // dx/2.0 ...
-    int boundary[6];       // +x -x +y -y +z -z : 1 escape, 2 reflect
+    int boundaryMask;      // 2 bits per face +x -x +y -y +z -z: 1 escape, 2 reflect
     const double* planes;
     const int32_t* cellMat;
     const double* cellVolume;

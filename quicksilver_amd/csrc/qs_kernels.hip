@@ -30,6 +30,10 @@ constexpr double kPi = 3.1415926535897932;
 constexpr uint32_t kMaxSegmentsPerHistory = 1u << 26;
 
 enum { kScatter = 1, kAbsorption = 2, kFission = 3 };
+enum : uint32_t {
+    kEvSegment = 1u, kEvCollision = 2u, kEvScatter = 4u, kEvAbsorb = 8u,
+    kEvFission = 16u, kEvEscape = 32u, kEvCensus = 64u
+};
 enum { kEscape = 1, kReflect = 2 };
 
 // ---------------------------------------------------------------- RNG ----
@@ -123,42 +127,77 @@ __device__ __forceinline__ Node3 facet_node(const Cell9& c, uint32_t code, int m
     return n;
 }
 
-// MCT_Nearest_Facet_3D_G_Distance_To_Segment (MCT.cc:280-395) for a facet
-// whose outward normal has a positive dot product with the direction.  The
-// reference's three projection branches (|C|>0.5, |B|>0.5, |A|>0.5) are
-// folded into a per-lane choice of the projected axes (u,v), which keeps the
-// wave converged; each branch checks the bounding box on exactly the two
-// projected axes, so only the selection differs, not the arithmetic.
-__device__ __forceinline__ double distance_to_facet(double plane_tol, double dot, double A, double B,
-                                                    double C, double D, const Node3& p0,
-                                                    const Node3& p1, const Node3& p2, double x,
-                                                    double y, double z, double a, double b, double g) {
+// Cell-local node triples of the 24 facets (MC_Domain.cc:42-47) resolved to
+// per-axis value codes of the corner tuples (GlobalFccGrid.cc:47-69):
+// 0 -> i*dx, 1 -> (i+1)*dx, 2 -> i*dx + dx/2.  kFacet.code[f][node][axis].
+struct FacetTable {
+    uint8_t code[24][3][3];
+};
+
+constexpr FacetTable makeFacetTable() {
+    const int corner[14][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {1, 1, 0, 0}, {0, 0, 1, 0},
+                               {1, 0, 1, 0}, {0, 1, 1, 0}, {1, 1, 1, 0}, {1, 0, 0, 1}, {0, 0, 0, 1},
+                               {0, 1, 0, 2}, {0, 0, 0, 2}, {0, 0, 1, 3}, {0, 0, 0, 3}};
+    const int tri[24][3] = {{1, 3, 8},  {3, 7, 8},  {7, 5, 8},  {5, 1, 8},  {0, 4, 9},  {4, 6, 9},
+                            {6, 2, 9},  {2, 0, 9},  {3, 2, 10}, {2, 6, 10}, {6, 7, 10}, {7, 3, 10},
+                            {0, 1, 11}, {1, 5, 11}, {5, 4, 11}, {4, 0, 11}, {4, 5, 12}, {5, 7, 12},
+                            {7, 6, 12}, {6, 4, 12}, {0, 2, 13}, {2, 3, 13}, {3, 1, 13}, {1, 0, 13}};
+    FacetTable t{};
+    for (int f = 0; f < 24; f++)
+        for (int m = 0; m < 3; m++) {
+            const int* o = corner[tri[f][m]];
+            t.code[f][m][0] = (uint8_t)((o[3] == 2 || o[3] == 3) ? 2 : o[0]);
+            t.code[f][m][1] = (uint8_t)((o[3] == 1 || o[3] == 3) ? 2 : o[1]);
+            t.code[f][m][2] = (uint8_t)((o[3] == 1 || o[3] == 2) ? 2 : o[2]);
+        }
+    return t;
+}
+
+constexpr FacetTable kFacet = makeFacetTable();
+
+// Value of one node coordinate for a compile-time code.
+template <int AXIS, int CODE>
+__device__ __forceinline__ double node_value(const Cell9& c) {
+    if (AXIS == 0) return CODE == 0 ? c.X0 : (CODE == 1 ? c.X1 : c.Xh);
+    if (AXIS == 1) return CODE == 0 ? c.Y0 : (CODE == 1 ? c.Y1 : c.Yh);
+    return CODE == 0 ? c.Z0 : (CODE == 1 ? c.Z1 : c.Zh);
+}
+
+// Node coordinate of facet FP (lane leaves through the + face) or FM (the -
+// face of the same axis).
+template <int FP, int FM, int M, int AXIS>
+__device__ __forceinline__ double node_coord(const Cell9& c, bool plus) {
+    double vp = node_value<AXIS, kFacet.code[FP][M][AXIS]>(c);
+    double vm = node_value<AXIS, kFacet.code[FM][M][AXIS]>(c);
+    return plus ? vp : vm;
+}
+
+// MCT_Nearest_Facet_3D_G_Distance_To_Segment (MCT.cc:280-395) for a facet of
+// a face normal to axis AX.  On this mesh every such plane has its other two
+// components exactly 0 and |component AX| > 0.5 (checked on the host when
+// the problem is built), so the reference takes the |A|>0.5 branch for x
+// faces, |B|>0.5 for y faces and |C|>0.5 for z faces; the branch, its
+// bounding-box axes and its projected (u,v) axes are therefore known here at
+// compile time: AX 0 -> (y,z), AX 1 -> (z,x), AX 2 -> (x,y).
+template <int AX>
+__device__ __forceinline__ double distance_to_facet(double planeTol, double dot, double A, double B, double C,
+                                                    double D, double p0u, double p0v, double p1u, double p1v,
+                                                    double p2u, double p2v, double x, double y, double z,
+                                                    double a, double b, double g) {
     const double tol = 1e-9;
     double numerator = -1.0 * (A * x + B * y + C * z + D);
-    if (numerator < 0.0 && numerator * numerator > plane_tol) return kHuge;
+    if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
     double distance = numerator / dot;
     double ix = x + distance * a;
     double iy = y + distance * b;
     double iz = z + distance * g;
-
-    int axis = (C < -0.5 || C > 0.5) ? 2 : ((B < -0.5 || B > 0.5) ? 1 : ((A < -0.5 || A > 0.5) ? 0 : -1));
-    if (axis < 0) return kHuge;
-    // axis 2: (u,v) = (x,y); axis 1: (z,x); axis 0: (y,z)
-    double p0u = axis == 2 ? p0.x : (axis == 1 ? p0.z : p0.y);
-    double p1u = axis == 2 ? p1.x : (axis == 1 ? p1.z : p1.y);
-    double p2u = axis == 2 ? p2.x : (axis == 1 ? p2.z : p2.y);
-    double iu = axis == 2 ? ix : (axis == 1 ? iz : iy);
-    double p0v = axis == 2 ? p0.y : (axis == 1 ? p0.x : p0.z);
-    double p1v = axis == 2 ? p1.y : (axis == 1 ? p1.x : p1.z);
-    double p2v = axis == 2 ? p2.y : (axis == 1 ? p2.x : p2.z);
-    double iv = axis == 2 ? iy : (axis == 1 ? ix : iz);
-
+    const double iu = AX == 0 ? iy : (AX == 1 ? iz : ix);
+    const double iv = AX == 0 ? iz : (AX == 1 ? ix : iy);
     double iuP = iu + tol, iuM = iu - tol, ivP = iv + tol, ivM = iv - tol;
     if (p0u > iuP && p1u > iuP && p2u > iuP) return kHuge;
     if (p0u < iuM && p1u < iuM && p2u < iuM) return kHuge;
     if (p0v > ivP && p1v > ivP && p2v > ivP) return kHuge;
     if (p0v < ivM && p1v < ivM && p2v < ivM) return kHuge;
-
     double cross1 = (p1u - p0u) * (iv - p0v) - (p1v - p0v) * (iu - p0u);
     double cross2 = (p2u - p1u) * (iv - p1v) - (p2v - p1v) * (iu - p1u);
     double cross0 = (p0u - p2u) * (iv - p2v) - (p0v - p2v) * (iu - p2u);
@@ -175,75 +214,112 @@ struct Nearest {
     int f;
 };
 
-// MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G + both Find_Nearest routines
-// (MCT.cc:87-137, 436-536, 550-621).  Facets facing away from the direction
-// (dot <= 0) keep distance kHuge; the candidates are visited in increasing
-// facet order and the skipped ranges are merged in with the same `<=` rule,
-// so ties and the "no positive distance" fallback resolve exactly as in the
-// reference.  May move (x,y,z) toward the cell centre and retry.
+// Running state of MCT_Nearest_Facet_Find_Nearest (MCT.cc:436-479): the
+// reference scans facets 0..23 with "d > 0 && d <= best" (ties go to the
+// later facet) and keeps the largest non-positive d (ties to the earlier).
+// Facets with distance kHuge (facing away, or rejected) are tracked as the
+// largest such index, so the scan can skip them and still end identically.
+struct FindNearest {
+    double pos;  // smallest positive distance other than kHuge
+    int posF;
+    int hugeF;   // largest facet index whose distance is kHuge
+    double neg;
+    int negF;
+    __device__ __forceinline__ void add(double d, int f) {
+        if (d == kHuge) {
+            hugeF = f > hugeF ? f : hugeF;
+        } else if (d > 0.0) {
+            if (d <= pos) {
+                pos = d;
+                posF = f;
+            }
+        } else if (d > neg) {
+            neg = d;
+            negF = f;
+        }
+    }
+    __device__ __forceinline__ Nearest result() const {
+        Nearest n;
+        n.d = 1e80;
+        n.f = 0;
+        if (posF >= 0 && !(hugeF >= 0 && pos > kHuge)) {
+            n.d = pos;
+            n.f = posF;
+        } else if (hugeF >= 0) {
+            n.d = kHuge;
+            n.f = hugeF;
+        } else if (posF >= 0) {
+            n.d = pos;
+            n.f = posF;
+        }
+        if (n.d == kHuge && neg != -kHuge) {
+            n.d = neg;
+            n.f = negF;
+        }
+        return n;
+    }
+};
+
+// The four facets of the face of axis AX the lane leaves through.
+template <int AX, int T>
+__device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
+                                            double x, double y, double z, double a, double b, double g, bool plus,
+                                            bool minus) {
+    constexpr int FP = 8 * AX + T, FM = FP + 4;
+    constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
+    constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
+    const int f = plus ? FP : FM;
+    if (!(plus || minus)) {
+        fn.add(kHuge, FP);
+        fn.add(kHuge, FM);
+        return;
+    }
+    const double* p = pl + 4 * f;
+    const double A = p[0], B = p[1], C = p[2], D = p[3];
+    const double dot = A * a + B * b + C * g;
+    double d = kHuge;
+    if (dot > 0.0) {
+        const double p0u = node_coord<FP, FM, 0, U>(c9, plus), p0v = node_coord<FP, FM, 0, V>(c9, plus);
+        const double p1u = node_coord<FP, FM, 1, U>(c9, plus), p1v = node_coord<FP, FM, 1, V>(c9, plus);
+        const double p2u = node_coord<FP, FM, 2, U>(c9, plus), p2v = node_coord<FP, FM, 2, V>(c9, plus);
+        d = distance_to_facet<AX>(planeTol, dot, A, B, C, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
+    }
+    // the opposite face faces away from the direction: distance kHuge
+    fn.add(kHuge, plus ? FM : FP);
+    fn.add(d, f);
+}
+
+template <int AX>
+__device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
+                                           double x, double y, double z, double a, double b, double g) {
+    const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
+    const bool plus = dAx > 0.0, minus = dAx < 0.0;
+    visit_facet<AX, 0>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 1>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 2>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 3>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+}
+
+// MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G (MCT.cc:87-137, 550-621) with
+// the retry of MCT_Nearest_Facet_Find_Nearest (MCT.cc:487-536): when no
+// facet bounds the particle it is nudged toward the cell centre and the
+// search repeats.  Only the 12 facets of the three faces the direction
+// points out of are tested; the other 12 have dot <= 0 and distance kHuge
+// in the reference too.
 __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell,
                                                  const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
-    uint32_t cand = 0;
-#pragma unroll
-    for (int f = 0; f < 24; f++) {
-        double A = pl[4 * f + 0], B = pl[4 * f + 1], C = pl[4 * f + 2];
-        double dot = A * a + B * b + C * g;
-        if (dot > 0.0) cand |= 1u << f;
-    }
     int iteration = 0;
     double moveFactor = 0.5 * kSmall;
     Nearest nf;
     for (;;) {
         double planeTol = 1e-16 * (x * x + y * y + z * z);
-        double nearD = 1e80;
-        int nearF = 0;
-        double negD = -kHuge;
-        int negF = 0;
-        int prev = -1;
-        uint32_t m = cand;
-        while (m) {
-            int f = __builtin_ctz(m);
-            m &= m - 1;
-            uint32_t gapMask = ~cand & (((1u << f) - 1u) & ~((prev >= 0) ? ((2u << prev) - 1u) : 0u));
-            if (gapMask && kHuge <= nearD) {
-                nearD = kHuge;
-                nearF = 31 - __builtin_clz(gapMask);
-            }
-            const double* p = pl + 4 * f;
-            double A = p[0], B = p[1], C = p[2], D = p[3];
-            double dot = A * a + B * b + C * g;
-            uint32_t code = c_facetCode[f];
-            Node3 n0 = facet_node(c9, code, 0);
-            Node3 n1 = facet_node(c9, code, 1);
-            Node3 n2 = facet_node(c9, code, 2);
-            double d = distance_to_facet(planeTol, dot, A, B, C, D, n0, n1, n2, x, y, z, a, b, g);
-            if (d > 0.0) {
-                if (d <= nearD) {
-                    nearD = d;
-                    nearF = f;
-                }
-            } else if (d > negD) {
-                negD = d;
-                negF = f;
-            }
-            prev = f;
-        }
-        {
-            uint32_t gapMask = ~cand & (0xFFFFFFu & ~((prev >= 0) ? ((2u << prev) - 1u) : 0u));
-            if (gapMask && kHuge <= nearD) {
-                nearD = kHuge;
-                nearF = 31 - __builtin_clz(gapMask);
-            }
-        }
-        if (nearD == kHuge && negD != -kHuge) {
-            nearD = negD;
-            nearF = negF;
-        }
-        nf.d = nearD;
-        nf.f = nearF;
-        if (nearD != kHuge) break;
-        // no facet bounds the particle: nudge toward the cell centre
+        FindNearest fn = {1e80, -1, -1, -kHuge, 0};
+        visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+        visit_axis<1>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+        visit_axis<2>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+        nf = fn.result();
+        if (nf.d != kHuge) break;
         const double* cc = P.cellCenter + 3 * (size_t)cell;
         x += moveFactor * (cc[0] - x);
         y += moveFactor * (cc[1] - y);
@@ -413,6 +489,8 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
     bool active = false;
     Lane L = {};
     uint32_t histSeg = 0;
+    // wave-uniform (scalar) balance counters, updated from ballots at the
+    // converged end of each iteration
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
 
     for (;;) {
@@ -472,8 +550,9 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
             }
         }
         if (__ballot(active) == 0) break;
-        if (!active) continue;
-
+        uint32_t ev = 0;  // per-lane event bits of this iteration
+        uint32_t nProd = 0;
+        if (active) {
         // ---- one segment: MC_Segment_Outcome (MC_Segment_Outcome.cc:31-226) ----
         const int cell = L.ci + P.nx * (L.cj + P.ny * L.ck);
         const int mat = P.cellMat[cell];
@@ -523,7 +602,7 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
             if (L.tc < 0.0) L.tc = 0.0;
             unsafeAtomicAdd(&P.flux[(size_t)cell * G + L.group], seg * L.w);
         }
-        cSeg++;
+        ev |= kEvSegment;
         histSeg++;
 
         if (outcome == 0) {
@@ -574,34 +653,40 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                     nOut = 4;
                 }
             }
-            cCol++;
-            if (type == kScatter) cScat++;
-            else if (type == kAbsorption) cAbs++;
+            ev |= kEvCollision;
+            if (type == kScatter) ev |= kEvScatter;
+            else if (type == kAbsorption) ev |= kEvAbsorb;
             else if (type == kFission) {
-                cFis++;
-                cProd += (uint32_t)nOut;
+                ev |= kEvFission;
+                nProd = (uint32_t)nOut;
             }
             if (nOut == 0) {
                 active = false;
             } else {
                 uint32_t slot = 0;
                 if (nOut > 1) slot = wave_append(nExtraOut, (uint32_t)nOut);
-                // secondaries 1..nOut-1: copies of the pre-collision parent
-                // with spawned seeds (CollisionEvent.cc:125-133)
-#pragma unroll
-                for (int k = 1; k < 4; k++) {
-                    if (k < nOut) {
-                        Lane S = L;
-                        S.seed = rng_spawn(L.seed);
-                        double ek = k == 1 ? e1 : (k == 2 ? e2 : e3);
-                        double ak = k == 1 ? a1 : (k == 2 ? a2 : a3);
-                        update_trajectory(S, ek, ak);
+                // secondaries k = 1..nOut-1 are copies of the pre-collision
+                // parent with spawned seeds (CollisionEvent.cc:125-133); the
+                // parent itself (k = nOut, outgoing pair 0) is updated last
+                // (CollisionEvent.cc:135).  One loop body keeps a single copy
+                // of updateTrajectory's sin/cos/log in the kernel.
+#pragma unroll 1
+                for (int k = 1; k <= nOut; k++) {
+                    const bool parent = (k == nOut);
+                    Lane S = L;
+                    if (!parent) S.seed = rng_spawn(L.seed);
+                    const int j = parent ? 0 : k;
+                    double ek = j == 0 ? e0 : (j == 1 ? e1 : (j == 2 ? e2 : e3));
+                    double ak = j == 0 ? a0 : (j == 1 ? a1 : (j == 2 ? a2 : a3));
+                    update_trajectory(S, ek, ak);
+                    if (parent) {
+                        L = S;
+                    } else {
                         uint32_t o = slot + (uint32_t)(k - 1);
                         if (o < extraCap) store_particle(extra, o, S, cell);
                         else atomicOr(&pc->error, 2u);
                     }
                 }
-                update_trajectory(L, e0, a0);
                 L.speed = vlength(L.vx, L.vy, L.vz);
                 if (nOut > 1) {
                     uint32_t o = slot + (uint32_t)(nOut - 1);
@@ -627,8 +712,8 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                 L.ci = ni;
                 L.cj = nj;
                 L.ck = nk;
-            } else if (P.boundary[face] == kEscape) {
-                cEsc++;
+            } else if (((P.boundaryMask >> (2 * face)) & 3) == kEscape) {
+                ev |= kEvEscape;
                 active = false;
             } else {
                 // MCT_Reflect_Particle (MCT.cc:401-429)
@@ -651,24 +736,26 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
             uint32_t o = wave_append(&pc->nCensus, 1u);
             if (o < censusCap) store_particle(census, o, L, cell);
             else atomicOr(&pc->error, 1u);
-            cCen++;
+            ev |= kEvCensus;
             active = false;
         }
         if (active && histSeg >= kMaxSegmentsPerHistory) {
             atomicOr(&pc->error, 8u);
             active = false;
         }
+        }  // active
+        cSeg += __popcll(__ballot(ev & kEvSegment));
+        cCol += __popcll(__ballot(ev & kEvCollision));
+        cScat += __popcll(__ballot(ev & kEvScatter));
+        cAbs += __popcll(__ballot(ev & kEvAbsorb));
+        cFis += __popcll(__ballot(ev & kEvFission));
+        cEsc += __popcll(__ballot(ev & kEvEscape));
+        cCen += __popcll(__ballot(ev & kEvCensus));
+        cProd += __popcll(__ballot(nProd > 0)) + __popcll(__ballot(nProd > 1)) +
+                 __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
     }
 
-    // ---- wave-reduced balance tallies ----
-    cSeg = wave_sum(cSeg);
-    cCol = wave_sum(cCol);
-    cScat = wave_sum(cScat);
-    cAbs = wave_sum(cAbs);
-    cFis = wave_sum(cFis);
-    cProd = wave_sum(cProd);
-    cEsc = wave_sum(cEsc);
-    cCen = wave_sum(cCen);
+    // ---- balance tallies: one 64-bit atomic per counter per wave ----
     if (lane == 0) {
         if (cSeg) atomicAdd(&dc->numSegments, (unsigned long long)cSeg);
         if (cCol) atomicAdd(&dc->collision, (unsigned long long)cCol);

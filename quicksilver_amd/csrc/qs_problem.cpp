@@ -256,6 +256,18 @@ ProblemTables buildProblem(const Parameters& params) {
                     B *= inv;
                     C *= inv;
                     D *= inv;
+                    // The device tracks with the facets of one face sharing a
+                    // projection axis: the face normal must be axis-aligned
+                    // (two components exactly zero) with the outward sign.
+                    {
+                        const int ax = f / 8;
+                        const double comp[3] = {A, B, C};
+                        const bool plus = (f % 8) < 4;
+                        bool ok = plus ? comp[ax] > 0.5 : comp[ax] < -0.5;
+                        for (int q = 0; q < 3; q++)
+                            if (q != ax && comp[q] != 0.0) ok = false;
+                        if (!ok) throw std::runtime_error("facet plane is not axis-aligned");
+                    }
                     double* p = &T.planes[(c * 24 + f) * 4];
                     p[0] = A;
                     p[1] = B;

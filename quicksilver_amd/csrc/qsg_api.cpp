@@ -203,7 +203,8 @@ void build(qsg_problem_s* p, int device) {
     d.hdx = T.dx / 2.0;
     d.hdy = T.dy / 2.0;
     d.hdz = T.dz / 2.0;
-    for (int f = 0; f < 6; f++) d.boundary[f] = T.boundary[f];
+    d.boundaryMask = 0;
+    for (int f = 0; f < 6; f++) d.boundaryMask |= (T.boundary[f] & 3) << (2 * f);
     d.planes = p->upload(T.planes);
     d.cellMat = p->upload(T.cellMaterial);
     d.cellVolume = p->upload(T.cellVolume);

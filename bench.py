@@ -51,6 +51,10 @@ def bytes_per_segment(stats: dict, n_iso: int, n_react: int) -> float:
             + 272 * loads)
 
 
+def progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(steps: int):
     if not os.path.exists(QS_REF):
         return None
@@ -76,6 +80,60 @@ def cpu_baseline(steps: int):
     return None
 
 
+def batched_vs_plain(n_problems: int):
+    """mixed_batch workload: the same decks one at a time vs in one batch;
+    per-problem tallies must be identical."""
+    from quicksilver_amd import autotune, decks, workloads
+    ds = decks.mixed_batch(count=n_problems)
+    texts = [d.text() for d in ds]
+    plain_tune = autotune.lookup(autotune.signature("plain", [open(DECK).read()]))
+    batch_tune = autotune.lookup(autotune.signature("batch", texts))
+    progress(f"mixed_batch: {n_problems} problems one at a time")
+    plain = workloads.run_one_at_a_time(ds, autotune.as_tuple(plain_tune) if plain_tune else None)
+    progress(f"mixed_batch: one at a time {plain['segments_per_s']:.4g} segments/s; batched")
+    batch = workloads.run_batched(ds, autotune.as_tuple(batch_tune) if batch_tune else None)
+    progress(f"mixed_batch: batched {batch['segments_per_s']:.4g} segments/s")
+    same = plain["balances"] == batch["balances"]
+    st = batch["stats"]
+    out = {
+        "workload": f"mixed_batch: {n_problems} problems, 1e4-2e6 particles (log-uniform), 8-48 cells per "
+                    "edge, 1-3 materials x 5-20 isotopes x 3 reactions, polynomial XS, 10 steps each",
+        "one_at_a_time_segments_per_s": plain["segments_per_s"],
+        "batched_segments_per_s": batch["segments_per_s"],
+        "speedup": batch["segments_per_s"] / plain["segments_per_s"],
+        "one_at_a_time_wall_s": plain["wall_s"],
+        "batched_wall_s": batch["wall_s"],
+        "segments": batch["segments"],
+        "tallies_identical": same,
+        "batched_kernel_segments_per_s": st["segments"] / st["track_kernel_seconds"]
+        if st["track_kernel_seconds"] > 0 else None,
+    }
+    if not same:
+        raise SystemExit("batched tallies differ from one-at-a-time tallies")
+    return out, n_problems / batch["wall_s"]
+
+
+def serving(n_requests: int, capacity_problems_per_s: float):
+    """serve_stream workload: Poisson arrivals at 0.75 x the batched capacity
+    (problems/s on mixed_batch), served with continuous batching and, as the
+    baseline, one at a time in arrival order."""
+    from quicksilver_amd import decks, workloads
+    rate = 0.75 * capacity_problems_per_s
+    reqs = decks.serve_stream(count=n_requests, rate_per_s=rate)
+    progress(f"serve_stream: {n_requests} requests at {rate:.3g}/s, continuous batching")
+    served = workloads.serve_stream(reqs, batched=True)
+    progress("serve_stream: one at a time")
+    base = workloads.serve_stream(reqs, batched=False)
+    keys = ("segments_per_s", "p50_s", "p99_s", "mean_s", "wall_s")
+    return {
+        "workload": f"serve_stream: {n_requests} requests drawn like mixed_batch with 5-20 steps, Poisson "
+                    f"arrivals at {rate:.3g}/s (0.75 x batched capacity), HBM budget 200 GiB",
+        "continuous_batching": {k: served[k] for k in keys},
+        "one_at_a_time": {k: base[k] for k in keys},
+        "throughput_ratio": served["segments_per_s"] / base["segments_per_s"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +141,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (mixed_batch) comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the problem is "
@@ -90,7 +152,12 @@ def main():
         sys.exit(2)
 
     import quicksilver_amd as q
+    from quicksilver_amd import autotune
+    shape = autotune.lookup(autotune.signature("plain", [open(DECK).read()]))
+    progress("nested_boxes: plain tracking")
     with q.Problem.from_file(DECK) as p:
+        if shape:
+            p.set_tuning(*autotune.as_tuple(shape))
         if args.warmup:
             p.run(args.warmup)
         p.reset_stats()
@@ -147,7 +214,12 @@ def main():
                           "wall": wall},
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        line["batched"], capacity = batched_vs_plain(args.batch_problems)
+        if not args.no_serve:
+            line["serving"] = serving(args.serve_requests, capacity)
     if not args.no_cpu_baseline:
+        progress("reference CPU path sample")
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line), flush=True)
 

@@ -131,6 +131,46 @@ int qsg_get_census(qsg_problem p, qsg_particles* out, uint64_t n);
 int qsg_get_processing_count(qsg_problem p, uint64_t* n);
 int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n);
 
+/* ---- Batched form ----
+ * Many independent problems (on one device, each at a cycle boundary) advance
+ * one cycle per qsg_batch_step through shared kernel launches.  Every problem
+ * keeps its own buffers, tallies and RNG streams, so its results are
+ * bit-identical to running it alone.  The problem handles stay valid and
+ * continue from the batch's state; they must not be advanced on their own
+ * while they belong to a batch.  No reference counterpart: the reference runs
+ * one problem per process. */
+typedef struct qsg_batch_s* qsg_batch;
+int qsg_batch_create(qsg_problem* problems, int n, qsg_batch* out);
+int qsg_batch_destroy(qsg_batch b);
+/* balances / flux_sums: arrays of n (may be NULL); inactive problems get 0 */
+int qsg_batch_step(qsg_batch b, qsg_balance* balances, double* flux_sums);
+int qsg_batch_set_active(qsg_batch b, int index, int active);
+int qsg_batch_set_tuning(qsg_batch b, const qsg_tuning* t);
+int qsg_batch_get_stats(qsg_batch b, qsg_stats* stats);
+
+/* ---- Serving loop ----
+ * Solve requests (deck text) are queued, admitted in arrival order while their
+ * device footprint fits hbm_budget_bytes (and at most max_active run, 0 = no
+ * cap), kept resident on the device, advanced together one cycle per
+ * qsg_server_step (a batch rebuilt whenever membership changes), and retired
+ * after their nSteps cycles.  A request's per-cycle tallies equal those of the
+ * same deck run alone.  No reference counterpart.
+ * Request states: 0 queued, 1 running, 2 done, 3 failed. */
+typedef struct qsg_server_s* qsg_server;
+typedef struct qsg_server_stats {
+    uint64_t submitted, admitted, completed, rejected, steps, segments;
+    uint64_t bytes_in_use, active, queued;
+} qsg_server_stats;
+int qsg_server_create(int device, uint64_t hbm_budget_bytes, int max_active, qsg_server* out);
+int qsg_server_destroy(qsg_server s);
+int qsg_server_submit(qsg_server s, const char* deck_text, uint64_t* request_id);
+int qsg_server_step(qsg_server s, int* n_active, int* n_queued);
+int qsg_server_status(qsg_server s, uint64_t id, int* state, int* cycles_done);
+int qsg_server_result(qsg_server s, uint64_t id, qsg_balance* balances, double* flux_sums, int n);
+int qsg_server_release(qsg_server s, uint64_t id);
+int qsg_server_set_tuning(qsg_server s, const qsg_tuning* t);
+int qsg_server_get_stats(qsg_server s, qsg_server_stats* stats);
+
 /* Device FP64 builtins on host arrays (rounding checks).
  * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a) */
 int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device);

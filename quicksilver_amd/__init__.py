@@ -29,7 +29,11 @@ EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy"
            "qsg_set_tuning", "qsg_get_tuning", "qsg_cycle_init", "qsg_cycle_tracking",
            "qsg_cycle_finalize", "qsg_run", "qsg_get_scalar_flux", "qsg_set_particles",
            "qsg_get_census_count", "qsg_get_census", "qsg_get_processing_count",
-           "qsg_get_processing", "qsg_math_probe", "qsg_get_stats", "qsg_reset_stats"]
+           "qsg_get_processing", "qsg_math_probe", "qsg_get_stats", "qsg_reset_stats",
+           "qsg_batch_create", "qsg_batch_destroy", "qsg_batch_step", "qsg_batch_set_active",
+           "qsg_batch_set_tuning", "qsg_batch_get_stats", "qsg_server_create", "qsg_server_destroy",
+           "qsg_server_submit", "qsg_server_step", "qsg_server_status", "qsg_server_result",
+           "qsg_server_release", "qsg_server_set_tuning", "qsg_server_get_stats"]
 
 
 class QsgError(RuntimeError):
@@ -64,6 +68,14 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "admitted", "completed", "rejected", "steps",
+                                               "segments", "bytes_in_use", "active", "queued")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class Tuning(ctypes.Structure):
@@ -159,6 +171,22 @@ def lib() -> ctypes.CDLL:
     L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
     L.qsg_get_stats.argtypes = [vp, _P(Stats)]
     L.qsg_reset_stats.argtypes = [vp]
+    L.qsg_batch_create.argtypes = [_P(vp), i, _P(vp)]
+    L.qsg_batch_destroy.argtypes = [vp]
+    L.qsg_batch_step.argtypes = [vp, _P(Balance), _D]
+    L.qsg_batch_set_active.argtypes = [vp, i, i]
+    L.qsg_batch_set_tuning.argtypes = [vp, _P(Tuning)]
+    L.qsg_batch_get_stats.argtypes = [vp, _P(Stats)]
+    u64 = ctypes.c_uint64
+    L.qsg_server_create.argtypes = [i, u64, i, _P(vp)]
+    L.qsg_server_destroy.argtypes = [vp]
+    L.qsg_server_submit.argtypes = [vp, c, _P(u64)]
+    L.qsg_server_step.argtypes = [vp, _P(i), _P(i)]
+    L.qsg_server_status.argtypes = [vp, u64, _P(i), _P(i)]
+    L.qsg_server_result.argtypes = [vp, u64, _P(Balance), _D, i]
+    L.qsg_server_release.argtypes = [vp, u64]
+    L.qsg_server_set_tuning.argtypes = [vp, _P(Tuning)]
+    L.qsg_server_get_stats.argtypes = [vp, _P(ServerStats)]
     _lib = L
     return L
 
@@ -283,6 +311,134 @@ class Problem:
         v = ps.c_view()
         _check(lib().qsg_get_processing(self._h, ctypes.byref(v), n.value))
         return ps
+
+
+class Batch:
+    """Many problems advancing one cycle per ``step()`` in shared launches
+    (qsg_batch).  The problems stay usable afterwards."""
+
+    def __init__(self, problems: Sequence[Problem]):
+        self.problems = list(problems)
+        arr = (ctypes.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
+        h = ctypes.c_void_p()
+        _check(lib().qsg_batch_create(arr, len(self.problems), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().qsg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self):
+        """One cycle of every active problem: list of (balance dict, flux sum)
+        (None for inactive problems)."""
+        n = len(self.problems)
+        bal = (Balance * n)()
+        flux = (ctypes.c_double * n)()
+        _check(lib().qsg_batch_step(self._h, bal, flux))
+        return [(bal[k].as_dict(), flux[k]) for k in range(n)]
+
+    def set_active(self, index: int, active: bool):
+        _check(lib().qsg_batch_set_active(self._h, index, 1 if active else 0))
+
+    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
+        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+        _check(lib().qsg_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(lib().qsg_batch_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def run(self, steps: Sequence[int]):
+        """Runs every problem for its own number of cycles (steps[i]),
+        deactivating each when it is done.  Returns per-problem lists of
+        (balance, flux sum)."""
+        out = [[] for _ in self.problems]
+        for k in range(max(steps)):
+            for i, s in enumerate(steps):
+                if k == s:
+                    self.set_active(i, False)
+            res = self.step()
+            for i, s in enumerate(steps):
+                if k < s:
+                    out[i].append(res[i])
+        return out
+
+
+class Server:
+    """Continuous-batching serving loop (qsg_server): requests are admitted by
+    HBM budget, stay resident, and advance together one cycle per step()."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 0, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().qsg_server_create(device, int(hbm_budget_bytes), max_active, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().qsg_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, deck_text: str) -> int:
+        rid = ctypes.c_uint64()
+        _check(lib().qsg_server_submit(self._h, deck_text.encode(), ctypes.byref(rid)))
+        return rid.value
+
+    def step(self):
+        a, qd = ctypes.c_int(), ctypes.c_int()
+        _check(lib().qsg_server_step(self._h, ctypes.byref(a), ctypes.byref(qd)))
+        return a.value, qd.value
+
+    def status(self, rid: int):
+        st, n = ctypes.c_int(), ctypes.c_int()
+        _check(lib().qsg_server_status(self._h, rid, ctypes.byref(st), ctypes.byref(n)))
+        return st.value, n.value
+
+    def result(self, rid: int):
+        _, n = self.status(rid)
+        bal = (Balance * max(n, 1))()
+        flux = (ctypes.c_double * max(n, 1))()
+        _check(lib().qsg_server_result(self._h, rid, bal, flux, n))
+        return [(bal[k].as_dict(), flux[k]) for k in range(n)]
+
+    def release(self, rid: int):
+        _check(lib().qsg_server_release(self._h, rid))
+
+    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
+        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+        _check(lib().qsg_server_set_tuning(self._h, ctypes.byref(t)))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().qsg_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
 
 
 def math_probe(op: int, a: np.ndarray, b: Optional[np.ndarray] = None, device: int = 0) -> np.ndarray:

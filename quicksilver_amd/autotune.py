@@ -1,0 +1,167 @@
+"""Autotuner for the tracking-pass launch shape.
+
+Knobs (qsg_tuning):
+  block_size    threads per workgroup (64..256)
+  waves_per_cu  persistent waves launched per compute unit
+  refill_min    idle lanes that make a wave refill from its queue
+                (how full a wave is kept with particles)
+  chunk         queue entries a wave claims per atomic
+
+Method: coordinate descent from the current best.  Each knob in turn is swept
+over its candidate values with the others fixed, scoring every candidate by
+the tracking-kernel throughput (segments per second of summed kernel time,
+HIP events on the library's stream) over a few cycles of the workload
+itself.  The sweep repeats until a full round changes nothing (at most
+``rounds`` rounds).  Results are cached in a JSON table keyed by a workload
+signature, so later runs start from the tuned shape.
+
+Usage:
+    python -m quicksilver_amd.autotune deck.inp [--batch deck2.inp ...]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+import quicksilver_amd as q
+
+SPACE = {
+    "block_size": [64, 128, 256],
+    "waves_per_cu": [4, 8, 12, 16, 24, 32],
+    "refill_min": [1, 4, 8, 16, 32],
+    "chunk": [16, 32, 64, 128, 256],
+}
+ORDER = ["waves_per_cu", "refill_min", "chunk", "block_size"]
+DEFAULT = {"block_size": 256, "waves_per_cu": 16, "refill_min": 16, "chunk": 64}
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+Shape = Dict[str, int]
+
+
+def as_tuple(s: Shape) -> Tuple[int, int, int, int]:
+    return (s["block_size"], s["waves_per_cu"], s["refill_min"], s["chunk"])
+
+
+def signature(kind: str, decks: Sequence[str]) -> str:
+    """Workload class: plain/batched, mesh size and population of the decks."""
+    import re
+    sizes = []
+    for text in decks:
+        nx = re.search(r"\bnx:\s*(\d+)", text)
+        npart = re.search(r"\bnParticles:\s*(\d+)", text)
+        sizes.append((int(nx.group(1)) if nx else 10, int(npart.group(1)) if npart else 1000000))
+    if kind == "plain":
+        n, p = sizes[0]
+        return f"plain:n{n}:p{p}"
+    tot = sum(p for _, p in sizes)
+    return f"batch:{len(sizes)}:p{tot}"
+
+
+def load_table(path: str = TABLE) -> Dict[str, Shape]:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(sig: str, path: str = TABLE) -> Optional[Shape]:
+    t = load_table(path)
+    if sig in t:
+        return {k: int(v) for k, v in t[sig].items() if k in DEFAULT}
+    return None
+
+
+def store(sig: str, shape: Shape, score: float, path: str = TABLE):
+    t = load_table(path)
+    t[sig] = {**shape, "segments_per_s": score, "tuned_at": time.strftime("%Y-%m-%d")}
+    with open(path, "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+
+
+def coordinate_descent(score: Callable[[Shape], float], start: Optional[Shape] = None, rounds: int = 2,
+                       log: Callable[[str], None] = print) -> Tuple[Shape, float]:
+    best = dict(start or DEFAULT)
+    best_score = score(best)
+    log(f"start {best} -> {best_score:.4g}")
+    seen = {tuple(sorted(best.items())): best_score}
+    for _ in range(rounds):
+        changed = False
+        for knob in ORDER:
+            for v in SPACE[knob]:
+                cand = dict(best, **{knob: v})
+                key = tuple(sorted(cand.items()))
+                if key in seen:
+                    continue
+                s = score(cand)
+                seen[key] = s
+                log(f"  {knob}={v}: {s:.4g}")
+                if s > best_score * 1.01:
+                    best, best_score, changed = cand, s, True
+        if not changed:
+            break
+    log(f"best {best} -> {best_score:.4g}")
+    return best, best_score
+
+
+def tune_plain(deck_text: str, warmup: int = 2, cycles: int = 2, rounds: int = 2,
+               log: Callable[[str], None] = print) -> Tuple[Shape, float]:
+    with q.Problem(deck_text) as p:
+        p.run(warmup)
+
+        def score(s: Shape) -> float:
+            p.set_tuning(*as_tuple(s))
+            p.reset_stats()
+            p.run(cycles)
+            st = p.stats()
+            return st["segments"] / st["track_kernel_seconds"] if st["track_kernel_seconds"] > 0 else 0.0
+
+        return coordinate_descent(score, lookup(signature("plain", [deck_text])), rounds, log)
+
+
+def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 1, rounds: int = 1,
+               log: Callable[[str], None] = print) -> Tuple[Shape, float]:
+    probs = [q.Problem(t) for t in deck_texts]
+    try:
+        with q.Batch(probs) as b:
+            for _ in range(warmup):
+                b.step()
+
+            def score(s: Shape) -> float:
+                b.set_tuning(*as_tuple(s))
+                st0 = b.stats()
+                for _ in range(cycles):
+                    b.step()
+                st = b.stats()
+                dt = st["track_kernel_seconds"] - st0["track_kernel_seconds"]
+                return (st["segments"] - st0["segments"]) / dt if dt > 0 else 0.0
+
+            return coordinate_descent(score, lookup(signature("batch", deck_texts)), rounds, log)
+    finally:
+        for p in probs:
+            p.close()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("decks", nargs="+")
+    ap.add_argument("--batch", action="store_true", help="tune the batched form over all decks")
+    ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--no-store", action="store_true")
+    a = ap.parse_args()
+    texts = [open(d).read() for d in a.decks]
+    if a.batch:
+        shape, sc = tune_batch(texts, rounds=a.rounds)
+        sig = signature("batch", texts)
+    else:
+        shape, sc = tune_plain(texts[0], rounds=a.rounds)
+        sig = signature("plain", texts)
+    if not a.no_store:
+        store(sig, shape, sc)
+    print(json.dumps({"signature": sig, "shape": shape, "segments_per_s": sc}))
+
+
+if __name__ == "__main__":
+    main()

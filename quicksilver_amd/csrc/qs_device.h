@@ -22,19 +22,26 @@
 
 namespace qs {
 
+// One particle buffer: twelve arrays carved from one allocation at a fixed
+// byte stride (>= capacity * 8, 256-byte aligned), in the order
+// x y z vx vy vz E w tc nmfp (f64), seed (u64), cell (i32).  A view is a
+// base pointer and the stride, so a kernel carries 4 registers per buffer.
 struct Particles {
-    double* x;
-    double* y;
-    double* z;
-    double* vx;
-    double* vy;
-    double* vz;
-    double* E;
-    double* w;
-    double* tc;    // time to census
-    double* nmfp;  // remaining mean free paths
-    uint64_t* seed;
-    int32_t* cell;
+    char* base;
+    uint64_t stride;
+    __host__ __device__ double* f(int k) const { return (double*)(base + (uint64_t)k * stride); }
+    __host__ __device__ double* x() const { return f(0); }
+    __host__ __device__ double* y() const { return f(1); }
+    __host__ __device__ double* z() const { return f(2); }
+    __host__ __device__ double* vx() const { return f(3); }
+    __host__ __device__ double* vy() const { return f(4); }
+    __host__ __device__ double* vz() const { return f(5); }
+    __host__ __device__ double* E() const { return f(6); }
+    __host__ __device__ double* w() const { return f(7); }
+    __host__ __device__ double* tc() const { return f(8); }     // time to census
+    __host__ __device__ double* nmfp() const { return f(9); }   // remaining mean free paths
+    __host__ __device__ uint64_t* seed() const { return (uint64_t*)(base + 10 * stride); }
+    __host__ __device__ int32_t* cell() const { return (int32_t*)(base + 11 * stride); }
 };
 
 constexpr int kParticleFields = 12;
@@ -101,6 +108,37 @@ struct TrackTuning {
     int chunk;         // particles a wave claims per queue access
 };
 
+// ---- batched form: many independent problems per launch ----
+// Per-problem mutable state of a batch (device memory).
+struct BatchState {
+    uint32_t head;      // pass queue head
+    uint32_t nCensus;   // census particles appended in the current cycle
+    uint32_t nPop;      // population-control output count
+    uint32_t error;     // same bits as PassCounters::error
+    uint32_t extra[2];  // secondaries appended by the last passes (ping-pong)
+    uint32_t start;     // census count carried into the current cycle
+    uint32_t nProc;     // start + source
+    uint32_t cycle;     // cycles completed; processing buffer = proc[cycle & 1]
+    uint32_t active;    // takes part in the next step
+    uint32_t mode;      // population control: 0 none, 1 roulette, 2 split
+    uint32_t pad;
+    double factor;      // population-control factor of the current cycle
+};
+
+// Per-problem read-only descriptor of a batch (device memory).
+struct BatchSlot {
+    DevProblem prob;
+    Particles proc[2];
+    Particles pop;
+    Particles extra[2];
+    uint32_t cap;           // capacity of each particle buffer
+    uint32_t loadBalance;
+    uint64_t sourcePerCycle;
+    uint64_t nParticles;    // population target
+    BatchState* st;
+    DevCounters* dc;
+};
+
 // ---- launchers (qs_kernels.hip); all asynchronous on `stream` ----
 struct LaunchCtx {
     void* stream;   // hipStream_t
@@ -133,6 +171,16 @@ void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partial
                    double* result);
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
+
+// Batched launchers over `n` slots.  prefix arrays hold n+1 entries.
+void launchBatchPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, uint64_t* srcPrefix, uint64_t* popPrefix);
+void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* srcPrefix);
+void launchBatchPopulation(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* popPrefix);
+// passKind 0: pop -> extra[0]; 1: extra[0] -> extra[1]; 2: extra[1] -> extra[0]
+void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind);
+void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune);
+void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,
+                         double* fluxSums);
 
 // Device copies of single FP64 builtins, for rounding checks.
 void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out, size_t n);

@@ -300,34 +300,69 @@ __device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pl, co
     visit_facet<AX, 3>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
 }
 
+// One facet of any face, for the rare retry path: facets are visited in
+// the reference's order 0..23 with the same arithmetic as visit_facet.
+__device__ __forceinline__ double distance_any_facet(const double* pl, const Cell9& c9, int f, double planeTol,
+                                                     double x, double y, double z, double a, double b,
+                                                     double g) {
+    const double* p = pl + 4 * f;
+    const double A = p[0], B = p[1], C = p[2], D = p[3];
+    const double dot = A * a + B * b + C * g;
+    if (dot <= 0.0) return kHuge;
+    const uint32_t code = c_facetCode[f];
+    const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
+    switch (f >> 3) {
+        case 0:
+            return distance_to_facet<0>(planeTol, dot, A, B, C, D, n0.y, n0.z, n1.y, n1.z, n2.y, n2.z, x, y, z, a,
+                                        b, g);
+        case 1:
+            return distance_to_facet<1>(planeTol, dot, A, B, C, D, n0.z, n0.x, n1.z, n1.x, n2.z, n2.x, x, y, z, a,
+                                        b, g);
+        default:
+            return distance_to_facet<2>(planeTol, dot, A, B, C, D, n0.x, n0.y, n1.x, n1.y, n2.x, n2.y, x, y, z, a,
+                                        b, g);
+    }
+}
+
 // MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G (MCT.cc:87-137, 550-621) with
 // the retry of MCT_Nearest_Facet_Find_Nearest (MCT.cc:487-536): when no
 // facet bounds the particle it is nudged toward the cell centre and the
-// search repeats.  Only the 12 facets of the three faces the direction
-// points out of are tested; the other 12 have dot <= 0 and distance kHuge
-// in the reference too.
+// search repeats.  The common first search tests only the 12 facets of the
+// three faces the direction points out of (the other 12 have dot <= 0 and
+// distance kHuge in the reference too), fully unrolled; the rare retries
+// use a compact loop over all 24 facets so they add no register pressure
+// to the common path.
 __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell,
                                                  const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
-    int iteration = 0;
-    double moveFactor = 0.5 * kSmall;
     Nearest nf;
-    for (;;) {
-        double planeTol = 1e-16 * (x * x + y * y + z * z);
+    {
+        const double planeTol = 1e-16 * (x * x + y * y + z * z);
         FindNearest fn = {1e80, -1, -1, -kHuge, 0};
         visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
         visit_axis<1>(fn, pl, c9, planeTol, x, y, z, a, b, g);
         visit_axis<2>(fn, pl, c9, planeTol, x, y, z, a, b, g);
         nf = fn.result();
-        if (nf.d != kHuge) break;
+    }
+    if (nf.d == kHuge) {
+        int iteration = 0;
+        double moveFactor = 0.5 * kSmall;
         const double* cc = P.cellCenter + 3 * (size_t)cell;
-        x += moveFactor * (cc[0] - x);
-        y += moveFactor * (cc[1] - y);
-        z += moveFactor * (cc[2] - z);
-        iteration++;
-        moveFactor *= 2.0;
-        if (moveFactor > 1.0e-2) moveFactor = 1.0e-2;
-        if (iteration == 10000) break;
+        for (;;) {
+            x += moveFactor * (cc[0] - x);
+            y += moveFactor * (cc[1] - y);
+            z += moveFactor * (cc[2] - z);
+            iteration++;
+            moveFactor *= 2.0;
+            if (moveFactor > 1.0e-2) moveFactor = 1.0e-2;
+            if (iteration == 10000) break;
+            const double planeTol = 1e-16 * (x * x + y * y + z * z);
+            FindNearest fn = {1e80, -1, -1, -kHuge, 0};
+#pragma unroll 1
+            for (int f = 0; f < 24; f++) fn.add(distance_any_facet(pl, c9, f, planeTol, x, y, z, a, b, g), f);
+            nf = fn.result();
+            if (nf.d != kHuge) break;
+        }
     }
     if (nf.d < 0) nf.d = 0;
     return nf;
@@ -388,18 +423,18 @@ __device__ __forceinline__ double vlength(double x, double y, double z) {
 }
 
 __device__ __forceinline__ void store_particle(const Particles& o, uint32_t i, const Lane& L, int cell) {
-    o.x[i] = L.x;
-    o.y[i] = L.y;
-    o.z[i] = L.z;
-    o.vx[i] = L.vx;
-    o.vy[i] = L.vy;
-    o.vz[i] = L.vz;
-    o.E[i] = L.E;
-    o.w[i] = L.w;
-    o.tc[i] = L.tc;
-    o.nmfp[i] = L.nmfp;
-    o.seed[i] = L.seed;
-    o.cell[i] = cell;
+    o.x()[i] = L.x;
+    o.y()[i] = L.y;
+    o.z()[i] = L.z;
+    o.vx()[i] = L.vx;
+    o.vy()[i] = L.vy;
+    o.vz()[i] = L.vz;
+    o.E()[i] = L.E;
+    o.w()[i] = L.w;
+    o.tc()[i] = L.tc;
+    o.nmfp()[i] = L.nmfp;
+    o.seed()[i] = L.seed;
+    o.cell()[i] = cell;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -466,21 +501,115 @@ __device__ __forceinline__ Tables stage_tables(const DevProblem& P, double* lds)
 
 // ------------------------------------------------------------- kernels ----
 
-template <bool kLds>
-__global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, Particles in,
-                                               const uint32_t* __restrict__ nInPtr, Particles census,
-                                               uint32_t censusCap, Particles extra, uint32_t* nExtraOut,
-                                               uint32_t extraCap, PassCounters* pc, DevCounters* dc) {
-    extern __shared__ double lds[];
-    __shared__ MatInfo sMat[kMaxMaterials];
-    __shared__ int32_t sReact[kMaxMaterials * kMaxReactions];
-    Tables T = stage_tables<kLds>(P, lds);
-    for (int q = threadIdx.x; q < P.nMat; q += blockDim.x) sMat[q] = P.mat[q];
-    for (int q = threadIdx.x; q < P.nMat * P.maxR; q += blockDim.x) sReact[q] = P.reactType[q];
-    __syncthreads();
+// Queue, outputs and counters of one problem in one tracking pass.
+struct QueueIO {
+    Particles in;
+    uint32_t nIn;
+    uint32_t* head;  // next queue entry handed to a wave
+    Particles census;
+    uint32_t* nCensus;
+    uint32_t censusCap;
+    Particles extra;
+    uint32_t* nExtra;
+    uint32_t extraCap;
+    uint32_t* error;
+    DevCounters* dc;
+};
 
-    const uint32_t nIn = *nInPtr;
-    const int G = P.G;
+// Accessors naming the queue a wave works on: one problem (k_track) or a
+// round-robin walk over the problems of a batch (k_batch_track).  Every
+// value they return is wave-uniform.
+struct SingleQueue {
+    static constexpr bool kMulti = false;
+    const DevProblem& P;
+    const Tables& T;
+    const QueueIO& io;
+    __device__ __forceinline__ const DevProblem& prob() const { return P; }
+    __device__ __forceinline__ Tables tables() const { return T; }
+    __device__ __forceinline__ QueueIO queue() const { return io; }
+    __device__ __forceinline__ bool next() { return false; }
+};
+
+struct BatchQueue {
+    static constexpr bool kMulti = true;
+    const BatchSlot* slots;
+    int n;
+    int passKind;
+    int p;
+    __device__ __forceinline__ const BatchSlot& slot() const { return slots[p]; }
+    __device__ __forceinline__ const DevProblem& prob() const { return slots[p].prob; }
+    __device__ __forceinline__ Tables tables() const {
+        const DevProblem& P = slots[p].prob;
+        Tables T;
+        T.energies = P.energies;
+        T.total = P.total;
+        T.macro = P.macro;
+        T.reactType = P.reactType;
+        T.mat = P.mat;
+        return T;
+    }
+    __device__ __forceinline__ uint32_t* inCount(int q) const {
+        BatchState* st = slots[q].st;
+        return passKind == 0 ? &st->nPop : &st->extra[passKind == 1 ? 0 : 1];
+    }
+    __device__ __forceinline__ QueueIO queue() const {
+        const BatchSlot& S = slots[p];
+        BatchState* st = S.st;
+        const int outIdx = passKind == 1 ? 1 : 0;
+        QueueIO io = {passKind == 0 ? S.pop : S.extra[passKind == 1 ? 0 : 1],
+                      (uint32_t)__builtin_amdgcn_readfirstlane(*inCount(p)),
+                      &st->head,
+                      S.proc[(st->cycle + 1) & 1],
+                      &st->nCensus,
+                      S.cap,
+                      S.extra[outIdx],
+                      &st->extra[outIdx],
+                      S.cap,
+                      &st->error,
+                      S.dc};
+        return io;
+    }
+    // Move to the next problem (round robin) whose queue still has entries.
+    __device__ __forceinline__ bool next() {
+        for (int tries = 0; tries < n; tries++) {
+            int q = (p + 1 == n) ? 0 : p + 1;
+            p = __builtin_amdgcn_readfirstlane(q);
+            BatchState* st = slots[p].st;
+            if (!st->active) continue;
+            const uint32_t nIn = (uint32_t)__builtin_amdgcn_readfirstlane(*inCount(p));
+            const uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane(*(volatile uint32_t*)&st->head);
+            if (h < nIn) return true;
+        }
+        return false;
+    }
+};
+
+// Adds a wave's balance counters to the problem's tallies (one 64-bit
+// atomic per nonzero counter, from lane 0) and clears them.
+__device__ __forceinline__ void flush_tallies(DevCounters* dc, uint32_t& cSeg, uint32_t& cCol, uint32_t& cScat,
+                                              uint32_t& cAbs, uint32_t& cFis, uint32_t& cProd, uint32_t& cEsc,
+                                              uint32_t& cCen) {
+    if (__lane_id() == 0) {
+        if (cSeg) atomicAdd(&dc->numSegments, (unsigned long long)cSeg);
+        if (cCol) atomicAdd(&dc->collision, (unsigned long long)cCol);
+        if (cScat) atomicAdd(&dc->scatter, (unsigned long long)cScat);
+        if (cAbs) atomicAdd(&dc->absorb, (unsigned long long)cAbs);
+        if (cFis) atomicAdd(&dc->fission, (unsigned long long)cFis);
+        if (cProd) atomicAdd(&dc->produce, (unsigned long long)cProd);
+        if (cEsc) atomicAdd(&dc->escape, (unsigned long long)cEsc);
+        if (cCen) atomicAdd(&dc->census, (unsigned long long)cCen);
+    }
+    cSeg = cCol = cScat = cAbs = cFis = cProd = cEsc = cCen = 0;
+}
+
+// Tracks every particle of the wave's queue(s) to the end of its history
+// in this pass (CycleTrackingGuts, CycleTracking.cc:15-119).  Idle lanes are
+// refilled from the queue; when the queue is drained the wave moves on to the
+// accessor's next queue, flushing its tallies first.
+template <class Q>
+__device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
+    QueueIO io = acc.queue();
+    uint32_t nIn = io.nIn;
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
 
@@ -494,13 +623,16 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
 
     for (;;) {
+        const DevProblem& P = acc.prob();
+        const Tables T = acc.tables();
+        const int G = P.G;
         // ---- refill idle lanes from the pass queue (wave-uniform control) ----
         uint64_t idle = __ballot(!active);
         if (!exhausted && idle != 0 && (__popcll(idle) >= (uint32_t)tune.refillMin || idle == ~0ull)) {
             while (idle != 0 && !exhausted) {
                 if (qBase >= qEnd) {
                     uint32_t got = 0;
-                    if (lane == 0) got = atomicAdd(&pc->head, (uint32_t)tune.chunk);
+                    if (lane == 0) got = atomicAdd(io.head, (uint32_t)tune.chunk);
                     got = __shfl(got, 0);
                     if (got >= nIn) {
                         exhausted = true;
@@ -516,18 +648,18 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                     // MC_Load_Particle (MC_Load_Particle.cc:11-29) +
                     // MC_Particle(const MC_Base_Particle&) (MC_Base_Particle.hh:281-320)
                     uint32_t idx = qBase + rank;
-                    L.x = in.x[idx];
-                    L.y = in.y[idx];
-                    L.z = in.z[idx];
-                    L.vx = in.vx[idx];
-                    L.vy = in.vy[idx];
-                    L.vz = in.vz[idx];
-                    L.E = in.E[idx];
-                    L.w = in.w[idx];
-                    L.tc = in.tc[idx];
-                    L.nmfp = in.nmfp[idx];
-                    L.seed = in.seed[idx];
-                    int cell = in.cell[idx];
+                    L.x = io.in.x()[idx];
+                    L.y = io.in.y()[idx];
+                    L.z = io.in.z()[idx];
+                    L.vx = io.in.vx()[idx];
+                    L.vy = io.in.vy()[idx];
+                    L.vz = io.in.vz()[idx];
+                    L.E = io.in.E()[idx];
+                    L.w = io.in.w()[idx];
+                    L.tc = io.in.tc()[idx];
+                    L.nmfp = io.in.nmfp()[idx];
+                    L.seed = io.in.seed()[idx];
+                    int cell = io.in.cell()[idx];
                     L.ci = cell % P.nx;
                     L.cj = (cell / P.nx) % P.ny;
                     L.ck = cell / (P.nx * P.ny);
@@ -549,7 +681,19 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                 idle = __ballot(!active);
             }
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active) == 0) {
+            // queue drained.  A single queue ends here (its tallies are
+            // flushed below the loop); a batch flushes this problem's
+            // tallies and moves on to the next queue.
+            if (!Q::kMulti) break;
+            flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
+            if (!acc.next()) break;
+            io = acc.queue();
+            nIn = io.nIn;
+            qBase = qEnd = 0;
+            exhausted = false;
+            continue;
+        }
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
         if (active) {
@@ -607,7 +751,7 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
 
         if (outcome == 0) {
             // ---- CollisionEvent (CollisionEvent.cc:50-148) ----
-            const MatInfo mi = sMat[mat];
+            const MatInfo mi = T.mat[mat];
             double r = rng_sample(L.seed);
             double cur = sigt * r;
             int selR = -1;
@@ -623,34 +767,28 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
             // The reference leaves the reaction undefined if rounding keeps
             // cur >= 0 after the full scan; the last reaction is used here.
             if (selR < 0) selR = mi.nReact - 1;
-            const int type = sReact[mat * P.maxR + selR];
-            // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88)
-            double e0 = 0, e1 = 0, e2 = 0, e3 = 0, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            const int type = T.reactType[mat * P.maxR + selR];
+            // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88).
+            // The outgoing (energy, angle) pairs are drawn here only to
+            // advance the seed; the loop below replays them from s0, so
+            // they are not held in registers across the secondaries.
+            const uint64_t s0 = L.seed;
             int nOut = 0;
             if (type == kScatter) {
                 nOut = 1;
-                r = rng_sample(L.seed);
-                e0 = L.E * (1.0 - (r * (1.0 / mi.mass)));
-                r = rng_sample(L.seed) * 2.0 - 1.0;
-                a0 = r;
+                rng_sample(L.seed);
+                rng_sample(L.seed);
             } else if (type == kFission) {
                 int n = (int)(mi.nuBar + rng_sample(L.seed));
                 nOut = n;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (k < n) {
-                        double rr = rng_sample(L.seed) / 2.0 + 0.5;
-                        double ek = (20 * rr * rr);
-                        rr = rng_sample(L.seed) * 2.0 - 1.0;
-                        if (k == 0) { e0 = ek; a0 = rr; }
-                        if (k == 1) { e1 = ek; a1 = rr; }
-                        if (k == 2) { e2 = ek; a2 = rr; }
-                        if (k == 3) { e3 = ek; a3 = rr; }
-                    }
-                }
                 if (nOut > 4) {
-                    atomicOr(&pc->error, 16u);  // MAX_PRODUCTION_SIZE exceeded
+                    atomicOr(io.error, 16u);  // MAX_PRODUCTION_SIZE exceeded
                     nOut = 4;
+                }
+#pragma unroll 1
+                for (int k = 0; k < nOut; k++) {
+                    rng_sample(L.seed);
+                    rng_sample(L.seed);
                 }
             }
             ev |= kEvCollision;
@@ -664,7 +802,7 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                 active = false;
             } else {
                 uint32_t slot = 0;
-                if (nOut > 1) slot = wave_append(nExtraOut, (uint32_t)nOut);
+                if (nOut > 1) slot = wave_append(io.nExtra, (uint32_t)nOut);
                 // secondaries k = 1..nOut-1 are copies of the pre-collision
                 // parent with spawned seeds (CollisionEvent.cc:125-133); the
                 // parent itself (k = nOut, outgoing pair 0) is updated last
@@ -676,22 +814,36 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
                     Lane S = L;
                     if (!parent) S.seed = rng_spawn(L.seed);
                     const int j = parent ? 0 : k;
-                    double ek = j == 0 ? e0 : (j == 1 ? e1 : (j == 2 ? e2 : e3));
-                    double ak = j == 0 ? a0 : (j == 1 ? a1 : (j == 2 ? a2 : a3));
+                    uint64_t t = s0;
+                    double ek, ak;
+                    if (type == kScatter) {
+                        ek = L.E * (1.0 - (rng_sample(t) * (1.0 / mi.mass)));
+                        ak = rng_sample(t) * 2.0 - 1.0;
+                    } else {
+                        rng_sample(t);  // the multiplicity draw
+#pragma unroll 1
+                        for (int i = 0; i < j; i++) {
+                            rng_sample(t);
+                            rng_sample(t);
+                        }
+                        double rr = rng_sample(t) / 2.0 + 0.5;
+                        ek = (20 * rr * rr);
+                        ak = rng_sample(t) * 2.0 - 1.0;
+                    }
                     update_trajectory(S, ek, ak);
                     if (parent) {
                         L = S;
                     } else {
                         uint32_t o = slot + (uint32_t)(k - 1);
-                        if (o < extraCap) store_particle(extra, o, S, cell);
-                        else atomicOr(&pc->error, 2u);
+                        if (o < io.extraCap) store_particle(io.extra, o, S, cell);
+                        else atomicOr(io.error, 2u);
                     }
                 }
                 L.speed = vlength(L.vx, L.vy, L.vz);
                 if (nOut > 1) {
                     uint32_t o = slot + (uint32_t)(nOut - 1);
-                    if (o < extraCap) store_particle(extra, o, L, cell);
-                    else atomicOr(&pc->error, 2u);
+                    if (o < io.extraCap) store_particle(io.extra, o, L, cell);
+                    else atomicOr(io.error, 2u);
                     active = false;
                 } else {
                     L.group = energy_group(T.energies, G, L.E);
@@ -733,14 +885,14 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
             }
         } else {
             // ---- census: processedVault->pushParticle (CycleTracking.cc:104-111) ----
-            uint32_t o = wave_append(&pc->nCensus, 1u);
-            if (o < censusCap) store_particle(census, o, L, cell);
-            else atomicOr(&pc->error, 1u);
+            uint32_t o = wave_append(io.nCensus, 1u);
+            if (o < io.censusCap) store_particle(io.census, o, L, cell);
+            else atomicOr(io.error, 1u);
             ev |= kEvCensus;
             active = false;
         }
         if (active && histSeg >= kMaxSegmentsPerHistory) {
-            atomicOr(&pc->error, 8u);
+            atomicOr(io.error, 8u);
             active = false;
         }
         }  // active
@@ -754,27 +906,37 @@ __global__ void __launch_bounds__(256) k_track(DevProblem P, TrackTuning tune, P
         cProd += __popcll(__ballot(nProd > 0)) + __popcll(__ballot(nProd > 1)) +
                  __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
     }
+    flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
 
-    // ---- balance tallies: one 64-bit atomic per counter per wave ----
-    if (lane == 0) {
-        if (cSeg) atomicAdd(&dc->numSegments, (unsigned long long)cSeg);
-        if (cCol) atomicAdd(&dc->collision, (unsigned long long)cCol);
-        if (cScat) atomicAdd(&dc->scatter, (unsigned long long)cScat);
-        if (cAbs) atomicAdd(&dc->absorb, (unsigned long long)cAbs);
-        if (cFis) atomicAdd(&dc->fission, (unsigned long long)cFis);
-        if (cProd) atomicAdd(&dc->produce, (unsigned long long)cProd);
-        if (cEsc) atomicAdd(&dc->escape, (unsigned long long)cEsc);
-        if (cCen) atomicAdd(&dc->census, (unsigned long long)cCen);
-    }
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_track(DevProblem P, TrackTuning tune, Particles in,
+                                               const uint32_t* __restrict__ nInPtr, Particles census,
+                                               uint32_t censusCap, Particles extra, uint32_t* nExtraOut,
+                                               uint32_t extraCap, PassCounters* pc, DevCounters* dc) {
+    extern __shared__ double lds[];
+    __shared__ MatInfo sMat[kMaxMaterials];
+    __shared__ int32_t sReact[kMaxMaterials * kMaxReactions];
+    Tables T = stage_tables<kLds>(P, lds);
+    for (int q = threadIdx.x; q < P.nMat; q += blockDim.x) sMat[q] = P.mat[q];
+    for (int q = threadIdx.x; q < P.nMat * P.maxR; q += blockDim.x) sReact[q] = P.reactType[q];
+    __syncthreads();
+    T.mat = sMat;
+    T.reactType = sReact;
+    QueueIO io = {in, *nInPtr, &pc->head, census, &pc->nCensus, censusCap, extra, nExtraOut, extraCap,
+                  &pc->error, dc};
+    SingleQueue acc = {P, T, io};
+    run_queue(acc, tune);
 }
 
 // MC_SourceNow particle generation (MC_SourceNow.cc:68-151) with
 // MCT_Generate_Coordinate_3D_G (MCT.cc:143-226) and Sample_Isotropic
 // (DirectionCosine.cc:5-13).
-__global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, uint32_t outBase,
-                                                uint64_t nSource, uint32_t cycle) {
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nSource) return;
+// Source particle number t (0-based within the problem's cycle) of cycle
+// `cycle`, written at out[o].
+__device__ __forceinline__ void source_particle(const DevProblem& P, const Particles& out, uint32_t o,
+                                                uint64_t t, uint32_t cycle) {
     // cell = last c with srcOffset[c] <= t
     int lo = 0, hi = P.nCells;  // srcOffset[lo] <= t < srcOffset[hi]
     while (hi - lo > 1) {
@@ -847,19 +1009,25 @@ __global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, uin
     double speed = kLight * sqrt(E * (E + 2.0 * (kRestMass)) / ((E + kRestMass) * (E + kRestMass)));
     double nmfp = -1.0 * log(rng_sample(seed));
     double tc = P.dt * rng_sample(seed);
-    const uint32_t o = outBase + (uint32_t)t;
-    out.x[o] = x;
-    out.y[o] = y;
-    out.z[o] = z;
-    out.vx[o] = speed * a;
-    out.vy[o] = speed * b;
-    out.vz[o] = speed * g;
-    out.E[o] = E;
-    out.w[o] = P.sourceWeight;
-    out.tc[o] = tc;
-    out.nmfp[o] = nmfp;
-    out.seed[o] = seed;
-    out.cell[o] = cell;
+    out.x()[o] = x;
+    out.y()[o] = y;
+    out.z()[o] = z;
+    out.vx()[o] = speed * a;
+    out.vy()[o] = speed * b;
+    out.vz()[o] = speed * g;
+    out.E()[o] = E;
+    out.w()[o] = P.sourceWeight;
+    out.tc()[o] = tc;
+    out.nmfp()[o] = nmfp;
+    out.seed()[o] = seed;
+    out.cell()[o] = cell;
+}
+
+__global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, uint32_t outBase,
+                                                uint64_t nSource, uint32_t cycle) {
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nSource) return;
+    source_particle(P, out, outBase + (uint32_t)t, t, cycle);
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -875,110 +1043,121 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 // PopulationControlGuts (PopulationControl.cc:71-122) followed by
 // RouletteLowWeightParticles (PopulationControl.cc:127-171), fused per
 // particle: every decision uses the particle's own RNG stream, so the
-// surviving multiset does not depend on processing order.
+// surviving multiset does not depend on processing order.  The outputs of
+// one input particle are the parent (its seed after all spawns) and
+// splitFactor copies (seed j = hash of the parent seed after j spawns); the
+// low-weight roulette then draws once from each output's own stream.
+struct PopDecision {
+    uint64_t seed;
+    double w;
+    int splitFactor;
+    bool keep;
+    bool roulette;
+    uint32_t survivors, kills, splits;
+};
+
+__device__ __forceinline__ PopDecision pop_decide(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
+                                                  int mode, double factor) {
+    PopDecision d = {0, 0.0, 0, valid, false, 0, 0, 0};
+    const double cutoff = P.lowWeightCutoff;
+    const double weightCutoff = cutoff * P.sourceWeight;
+    if (valid) {
+        d.seed = in.seed()[i];
+        d.w = in.w()[i];
+        if (mode == 1) {
+            double r = rng_sample(d.seed);
+            if (r > factor) {
+                d.keep = false;
+                d.kills++;
+            } else {
+                d.w /= factor;
+            }
+        } else if (mode == 2) {
+            double r = rng_sample(d.seed);
+            d.splitFactor = (int)floor(factor);
+            if (r > (factor - d.splitFactor)) d.splitFactor--;
+            d.w /= factor;
+            d.splits += (uint32_t)d.splitFactor;
+        }
+    }
+    d.roulette = cutoff > 0.0 && d.w <= weightCutoff;
+    if (d.keep) {
+        uint64_t s = d.seed;
+        for (int j = 0; j < d.splitFactor; j++) {
+            uint64_t cs = rng_spawn(s);
+            if (d.roulette) {
+                if (rng_sample(cs) <= cutoff) d.survivors++;
+                else d.kills++;
+            } else {
+                d.survivors++;
+            }
+        }
+        uint64_t ps = s;
+        if (d.roulette) {
+            if (rng_sample(ps) <= cutoff) d.survivors++;
+            else d.kills++;
+        } else {
+            d.survivors++;
+        }
+    }
+    return d;
+}
+
+__device__ __forceinline__ void pop_emit(const DevProblem& P, const Particles& in, uint32_t i, const PopDecision& d,
+                                         const Particles& out, uint32_t o, uint32_t outCap, uint32_t* error) {
+    const double cutoff = P.lowWeightCutoff;
+    const double x = in.x()[i], y = in.y()[i], z = in.z()[i], vx = in.vx()[i], vy = in.vy()[i], vz = in.vz()[i];
+    const double E = in.E()[i], tc = in.tc()[i], nmfp = in.nmfp()[i];
+    const int32_t cell = in.cell()[i];
+    const double wOut = d.roulette ? d.w / cutoff : d.w;
+    uint64_t s = d.seed;
+    auto emit = [&](uint64_t os) {
+        if (o < outCap) {
+            out.x()[o] = x;
+            out.y()[o] = y;
+            out.z()[o] = z;
+            out.vx()[o] = vx;
+            out.vy()[o] = vy;
+            out.vz()[o] = vz;
+            out.E()[o] = E;
+            out.w()[o] = wOut;
+            out.tc()[o] = tc;
+            out.nmfp()[o] = nmfp;
+            out.seed()[o] = os;
+            out.cell()[o] = cell;
+        } else {
+            atomicOr(error, 4u);
+        }
+        o++;
+    };
+    for (int j = 0; j < d.splitFactor; j++) {
+        uint64_t cs = rng_spawn(s);
+        if (d.roulette) {
+            uint64_t t = cs;
+            if (rng_sample(t) <= cutoff) emit(t);
+        } else {
+            emit(cs);
+        }
+    }
+    if (d.roulette) {
+        uint64_t t = s;
+        if (rng_sample(t) <= cutoff) emit(t);
+    } else {
+        emit(s);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
                                                     Particles out, uint32_t outCap, int mode,
                                                     double factor, PassCounters* pc, DevCounters* dc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const double cutoff = P.lowWeightCutoff;
-    const double weightCutoff = cutoff * P.sourceWeight;
-    uint64_t seed = 0;
-    double w = 0;
-    int splitFactor = 0;
-    uint32_t kills = 0, splits = 0;
-    bool keep = valid;
-    if (valid) {
-        seed = in.seed[i];
-        w = in.w[i];
-        if (mode == 1) {
-            double r = rng_sample(seed);
-            if (r > factor) {
-                keep = false;
-                kills++;
-            } else {
-                w /= factor;
-            }
-        } else if (mode == 2) {
-            double r = rng_sample(seed);
-            splitFactor = (int)floor(factor);
-            if (r > (factor - splitFactor)) splitFactor--;
-            w /= factor;
-            splits += (uint32_t)splitFactor;
-        }
-    }
-    // outputs: the parent (seed after all spawns) and splitFactor copies
-    // (seed j = hash of the parent seed after j spawns); low-weight roulette
-    // then draws once from each output's own stream.
-    const bool roulette = cutoff > 0.0 && w <= weightCutoff;
-    uint32_t survivors = 0;
-    if (keep) {
-        uint64_t s = seed;
-        for (int j = 0; j < splitFactor; j++) {
-            uint64_t cs = rng_spawn(s);
-            if (roulette) {
-                if (rng_sample(cs) <= cutoff) survivors++;
-                else kills++;
-            } else {
-                survivors++;
-            }
-        }
-        uint64_t ps = s;
-        if (roulette) {
-            if (rng_sample(ps) <= cutoff) survivors++;
-            else kills++;
-        } else {
-            survivors++;
-        }
-    }
-    uint32_t incl = wave_inclusive_scan(survivors);
+    PopDecision d = pop_decide(P, in, i, i < n, mode, factor);
+    uint32_t incl = wave_inclusive_scan(d.survivors);
     uint32_t base = 0;
     if (__lane_id() == 63 && incl) base = atomicAdd(&pc->nPopOut, incl);
     base = __shfl(base, 63);
-    uint32_t o = base + incl - survivors;
-    if (keep && survivors) {
-        const double x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-        const double E = in.E[i], tc = in.tc[i], nmfp = in.nmfp[i];
-        const int32_t cell = in.cell[i];
-        const double wOut = roulette ? w / cutoff : w;
-        uint64_t s = seed;
-        auto emit = [&](uint64_t os) {
-            if (o < outCap) {
-                out.x[o] = x;
-                out.y[o] = y;
-                out.z[o] = z;
-                out.vx[o] = vx;
-                out.vy[o] = vy;
-                out.vz[o] = vz;
-                out.E[o] = E;
-                out.w[o] = wOut;
-                out.tc[o] = tc;
-                out.nmfp[o] = nmfp;
-                out.seed[o] = os;
-                out.cell[o] = cell;
-            } else {
-                atomicOr(&pc->error, 4u);
-            }
-            o++;
-        };
-        for (int j = 0; j < splitFactor; j++) {
-            uint64_t cs = rng_spawn(s);
-            if (roulette) {
-                uint64_t t = cs;
-                if (rng_sample(t) <= cutoff) emit(t);
-            } else {
-                emit(cs);
-            }
-        }
-        if (roulette) {
-            uint64_t t = s;
-            if (rng_sample(t) <= cutoff) emit(t);
-        } else {
-            emit(s);
-        }
-    }
-    kills = wave_sum(kills);
-    splits = wave_sum(splits);
+    if (d.keep && d.survivors) pop_emit(P, in, i, d, out, base + incl - d.survivors, outCap, &pc->error);
+    uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
     if (__lane_id() == 0) {
         if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
         if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
@@ -1035,6 +1214,179 @@ __global__ void k_math(int op, const double* a, const double* b, double* out, si
         default: r = 0;
     }
     out[q] = r;
+}
+
+// ------------------------------------------------------ batched form ----
+// Many independent problems advance one cycle together.  Every problem keeps
+// its own particle buffers, tallies and RNG streams, so its results are the
+// same alone or in any mix.
+
+// Per-problem cycle set-up (the bookkeeping of main.cc:96-121 and
+// PopulationControl.cc:20-65): carried census count, source count,
+// population-control factor and counter resets; plus the exclusive prefix
+// sums over problems that map a flat thread index to (problem, particle).
+__global__ void k_batch_prep(BatchSlot* slots, int n, uint64_t* srcPrefix, uint64_t* popPrefix) {
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        BatchSlot& S = slots[p];
+        BatchState* st = S.st;
+        DevCounters* dc = S.dc;
+        const bool act = st->active != 0;
+        const uint32_t start = act ? st->nCensus : 0;
+        // census + source must fit the processing buffer; a problem that
+        // does not sources and tracks nothing and reports the overflow
+        const bool fits = (uint64_t)start + S.sourcePerCycle <= (uint64_t)S.cap;
+        st->start = start;
+        st->nProc = (act && fits) ? start + (uint32_t)S.sourcePerCycle : 0;
+        const uint32_t nProc = st->nProc;
+        const double target = (double)S.nParticles;
+        double factor;
+        if (S.loadBalance) factor = (nProc != 0) ? target / (double)nProc : 1.0;
+        else factor = target / (double)nProc;
+        uint32_t mode = 0;
+        if (factor != 1.0 && nProc != 0) mode = (factor < 1) ? 1u : ((factor > 1) ? 2u : 0u);
+        st->factor = factor;
+        st->mode = mode;
+        st->head = 0;
+        st->nPop = 0;
+        st->extra[0] = 0;
+        st->extra[1] = 0;
+        st->error = (act && !fits) ? 4u : 0u;
+        if (act) {
+            st->nCensus = 0;
+            unsigned long long* c = (unsigned long long*)dc;
+            for (int q = 0; q < (int)(sizeof(DevCounters) / 8); q++) c[q] = 0ull;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b = 0;
+        for (int p = 0; p < n; p++) {
+            srcPrefix[p] = a;
+            popPrefix[p] = b;
+            if (slots[p].st->active && slots[p].st->error == 0) a += slots[p].sourcePerCycle;
+            b += slots[p].st->nProc;
+        }
+        srcPrefix[n] = a;
+        popPrefix[n] = b;
+    }
+}
+
+__device__ __forceinline__ int find_problem(const uint64_t* prefix, int n, uint64_t t) {
+    int lo = 0, hi = n;  // prefix[lo] <= t < prefix[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_batch_source(BatchSlot* slots, int n, const uint64_t* srcPrefix) {
+    const uint64_t total = srcPrefix[n];
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int p = find_problem(srcPrefix, n, t);
+        const BatchSlot& S = slots[p];
+        const BatchState* st = S.st;
+        const uint64_t local = t - srcPrefix[p];
+        source_particle(S.prob, S.proc[st->cycle & 1], st->start + (uint32_t)local, local, st->cycle);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_population(BatchSlot* slots, int n, const uint64_t* popPrefix) {
+    const uint64_t total = popPrefix[n];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // all lanes of a wave run the same number of iterations
+    const uint64_t waveBase = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint64_t tw = waveBase; tw < total; tw += stride) {
+        const uint64_t t = tw + __lane_id();
+        const bool valid = t < total;
+        const int p = find_problem(popPrefix, n, valid ? t : total - 1);
+        const BatchSlot& S = slots[p];
+        BatchState* st = S.st;
+        const uint32_t i = (uint32_t)((valid ? t : total - 1) - popPrefix[p]);
+        PopDecision d = pop_decide(S.prob, S.proc[st->cycle & 1], i, valid, (int)st->mode, st->factor);
+        const int p0 = __shfl(p, 0);
+        const bool uniform = __ballot(p != p0) == 0;
+        uint32_t o;
+        if (uniform) {
+            uint32_t incl = wave_inclusive_scan(d.survivors);
+            uint32_t base = 0;
+            if (__lane_id() == 63 && incl) base = atomicAdd(&st->nPop, incl);
+            base = __shfl(base, 63);
+            o = base + incl - d.survivors;
+            uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
+            if (__lane_id() == 0) {
+                if (kills) atomicAdd(&S.dc->rr, (unsigned long long)kills);
+                if (splits) atomicAdd(&S.dc->split, (unsigned long long)splits);
+            }
+        } else {
+            o = d.survivors ? atomicAdd(&st->nPop, d.survivors) : 0u;
+            if (d.kills) atomicAdd(&S.dc->rr, (unsigned long long)d.kills);
+            if (d.splits) atomicAdd(&S.dc->split, (unsigned long long)d.splits);
+        }
+        if (d.keep && d.survivors)
+            pop_emit(S.prob, S.proc[st->cycle & 1], i, d, S.pop, o, S.cap, &st->error);
+    }
+}
+
+__global__ void k_batch_pass_prep(BatchSlot* slots, int n, int passKind) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        BatchState* st = slots[p].st;
+        st->head = 0;
+        st->extra[passKind == 1 ? 1 : 0] = 0;
+    }
+}
+
+// One tracking pass over every problem.  Each wave starts at its own
+// problem and moves round-robin through the problems, draining one
+// problem's queue at a time (so its descriptor and tables stay
+// wave-uniform); it exits after a full round finds no queued particle.
+__global__ void __launch_bounds__(256)
+k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    BatchQueue acc = {slots, n, passKind, (int)(wave % (uint32_t)n)};
+    if (!slots[acc.p].st->active && !acc.next()) return;
+    run_queue(acc, tune);
+}
+
+// Flux sum per problem in a fixed order (Tallies.cc:146-171), flux reset
+// (Tallies.cc:95), and the cycle advance of every active problem.
+__global__ void __launch_bounds__(kFluxBlock) k_batch_flux_partial(BatchSlot* slots, int partsPerProblem,
+                                                                   double* partials) {
+    __shared__ double sh[kFluxBlock];
+    const int p = blockIdx.x / partsPerProblem, part = blockIdx.x % partsPerProblem;
+    const BatchSlot& S = slots[p];
+    double acc = 0.0;
+    if (S.st->active) {
+        const size_t nf = (size_t)S.prob.nCells * S.prob.G;
+        const size_t per = (nf + partsPerProblem - 1) / partsPerProblem;
+        const size_t lo = (size_t)part * per;
+        const size_t hi = lo + per < nf ? lo + per : nf;
+        for (size_t q = lo + threadIdx.x; q < hi; q += kFluxBlock) {
+            acc += S.prob.flux[q];
+            S.prob.flux[q] = 0.0;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ void k_batch_flux_final(BatchSlot* slots, int n, int partsPerProblem, const double* partials,
+                                   double* fluxSums) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        BatchState* st = slots[p].st;
+        if (!st->active) continue;
+        double acc = 0.0;
+        for (int q = 0; q < partsPerProblem; q++) acc += partials[p * partsPerProblem + q];
+        fluxSums[p] = acc;
+        st->cycle++;
+    }
 }
 
 bool g_codesReady = false;
@@ -1134,6 +1486,44 @@ void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {
     if (blocks > 1024) blocks = 1024;
     if (blocks == 0) return;
     hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, s, (uint32_t*)ptr, words);
+}
+
+void launchBatchPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, uint64_t* srcPrefix, uint64_t* popPrefix) {
+    hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(1024), 0, (hipStream_t)ctx.stream, slots, n, srcPrefix, popPrefix);
+}
+
+void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* srcPrefix) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    ensureCodes(s);
+    hipLaunchKernelGGL(k_batch_source, dim3(ctx.numCUs * 8), dim3(256), 0, s, slots, n, srcPrefix);
+}
+
+void launchBatchPopulation(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* popPrefix) {
+    hipLaunchKernelGGL(k_batch_population, dim3(ctx.numCUs * 8), dim3(256), 0, (hipStream_t)ctx.stream, slots, n,
+                       popPrefix);
+}
+
+void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind) {
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_batch_pass_prep, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, slots, n, passKind);
+}
+
+void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune) {
+    const int wavesPerBlock = tune.blockSize / 64;
+    long long blocks = (long long)ctx.numCUs * tune.wavesPerCU / wavesPerBlock;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_batch_track, dim3((unsigned)blocks), dim3(tune.blockSize), 0, (hipStream_t)ctx.stream,
+                       slots, n, passKind, tune);
+}
+
+void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,
+                         double* fluxSums) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    hipLaunchKernelGGL(k_batch_flux_partial, dim3((unsigned)(n * partsPerProblem)), dim3(kFluxBlock), 0, s, slots,
+                       partsPerProblem, partials);
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_batch_flux_final, dim3(blocks), dim3(256), 0, s, slots, n, partsPerProblem, partials,
+                       fluxSums);
 }
 
 void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out,

@@ -24,112 +24,34 @@
 #include "qs_problem.h"
 #include "qsg.h"
 
+#include "qsg_internal.h"
+
+using qsgi::hipCheck;
+using qsgi::PBuf;
+using qsgi::QsgError;
+
 namespace {
-
 thread_local std::string g_lastError;
-
-struct QsgError : std::runtime_error {
-    int code;
-    QsgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void hipCheck(hipError_t e, const char* what) {
-    if (e != hipSuccess)
-        throw QsgError(QSG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-struct PBuf {
-    void* mem = nullptr;
-    qs::Particles v = {};
-    uint32_t cap = 0;
-};
-
 }  // namespace
 
-struct qsg_problem_s {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int numCUs = 0;
-    qs::Parameters params;
-    qs::ProblemTables tables;
-    qs::DevProblem dp = {};
-    std::vector<void*> allocs;
-    size_t deviceBytes = 0;
-
-    PBuf proc[2], pop, extra[2];
-    int cur = 0;               // proc[cur] holds this cycle's processing population
-    uint64_t startCount = 0;   // census particles carried into this cycle
-    uint32_t nPop = 0;         // processing population after population control
-    uint32_t nCensus = 0;      // census count after tracking
-    int censusBuf = -1;        // proc index holding the last census
-    int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
-    int cycle = 0;
-
-    qs::PassCounters* pc = nullptr;
-    qs::DevCounters* dc = nullptr;
-    qs::PassCounters* hpc = nullptr;  // pinned host mirrors
-    qs::DevCounters* hdc = nullptr;
-    double* partials = nullptr;
-    int nPartials = 0;
-    double* fluxSumDev = nullptr;
-    double* hFluxSum = nullptr;
-
-    qs::TrackTuning tune = {256, 16, 16, 64};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evI0 = nullptr, evI1 = nullptr, evF0 = nullptr, evF1 = nullptr;
-    std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
-    double lastTrackingSeconds = 0.0;
-    qsg_stats stats = {};
-
-    qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
-
-    template <typename T>
-    T* dalloc(size_t count) {
-        void* p = nullptr;
-        size_t bytes = count * sizeof(T);
-        if (bytes == 0) bytes = 8;
-        hipCheck(hipMalloc(&p, bytes), "hipMalloc");
-        allocs.push_back(p);
-        deviceBytes += bytes;
-        return static_cast<T*>(p);
-    }
-    template <typename T>
-    T* upload(const std::vector<T>& h) {
-        T* d = dalloc<T>(h.size());
-        if (!h.empty())
-            hipCheck(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
-        return d;
-    }
-};
+int qsgi::fail(const std::exception& e) {
+    g_lastError = e.what();
+    if (const QsgError* q = dynamic_cast<const QsgError*>(&e)) return q->code;
+    return QSG_ERR_ARG;
+}
 
 namespace {
 
-// Carve the 12 SoA arrays out of one allocation.
+// Carve the 12 SoA arrays out of one allocation at a fixed stride.
 PBuf makeParticles(qsg_problem_s* p, uint32_t cap) {
     PBuf b;
     b.cap = cap;
     const size_t n = cap ? cap : 1;
-    auto padded = [](size_t bytes) { return ((bytes + 255) / 256) * 256; };
-    size_t total = 10 * padded(n * 8) + padded(n * 8) + padded(n * 4);
-    char* m = p->dalloc<char>(total);
+    const size_t stride = ((n * 8 + 255) / 256) * 256;
+    char* m = p->dalloc<char>(12 * stride);
     b.mem = m;
-    size_t off = 0;
-    auto take = [&](size_t elem) {
-        char* q = m + off;
-        off += padded(n * elem);
-        return q;
-    };
-    b.v.x = (double*)take(8);
-    b.v.y = (double*)take(8);
-    b.v.z = (double*)take(8);
-    b.v.vx = (double*)take(8);
-    b.v.vy = (double*)take(8);
-    b.v.vz = (double*)take(8);
-    b.v.E = (double*)take(8);
-    b.v.w = (double*)take(8);
-    b.v.tc = (double*)take(8);
-    b.v.nmfp = (double*)take(8);
-    b.v.seed = (uint64_t*)take(8);
-    b.v.cell = (int32_t*)take(4);
+    b.v.base = m;
+    b.v.stride = stride;
     return b;
 }
 
@@ -141,15 +63,15 @@ void copyParticles(const qs::Particles& d, const qsg_particles* h, uint64_t n, h
     const bool up = kind == hipMemcpyHostToDevice;
     const size_t n8 = (size_t)n * 8, n4 = (size_t)n * 4;
     if (up) {
-        cp(d.x, h->x, n8); cp(d.y, h->y, n8); cp(d.z, h->z, n8);
-        cp(d.vx, h->vx, n8); cp(d.vy, h->vy, n8); cp(d.vz, h->vz, n8);
-        cp(d.E, h->energy, n8); cp(d.w, h->weight, n8); cp(d.tc, h->time_to_census, n8);
-        cp(d.nmfp, h->num_mean_free_paths, n8); cp(d.seed, h->seed, n8); cp(d.cell, h->cell, n4);
+        cp(d.x(), h->x, n8); cp(d.y(), h->y, n8); cp(d.z(), h->z, n8);
+        cp(d.vx(), h->vx, n8); cp(d.vy(), h->vy, n8); cp(d.vz(), h->vz, n8);
+        cp(d.E(), h->energy, n8); cp(d.w(), h->weight, n8); cp(d.tc(), h->time_to_census, n8);
+        cp(d.nmfp(), h->num_mean_free_paths, n8); cp(d.seed(), h->seed, n8); cp(d.cell(), h->cell, n4);
     } else {
-        cp(h->x, d.x, n8); cp(h->y, d.y, n8); cp(h->z, d.z, n8);
-        cp(h->vx, d.vx, n8); cp(h->vy, d.vy, n8); cp(h->vz, d.vz, n8);
-        cp(h->energy, d.E, n8); cp(h->weight, d.w, n8); cp(h->time_to_census, d.tc, n8);
-        cp(h->num_mean_free_paths, d.nmfp, n8); cp(h->seed, d.seed, n8); cp(h->cell, d.cell, n4);
+        cp(h->x, d.x(), n8); cp(h->y, d.y(), n8); cp(h->z, d.z(), n8);
+        cp(h->vx, d.vx(), n8); cp(h->vy, d.vy(), n8); cp(h->vz, d.vz(), n8);
+        cp(h->energy, d.E(), n8); cp(h->weight, d.w(), n8); cp(h->time_to_census, d.tc(), n8);
+        cp(h->num_mean_free_paths, d.nmfp(), n8); cp(h->seed, d.seed(), n8); cp(h->cell, d.cell(), n4);
     }
     hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
@@ -241,12 +163,7 @@ void build(qsg_problem_s* p, int device) {
     hipCheck(hipMemset(d.flux, 0, (size_t)T.nCells * T.nGroups * sizeof(double)), "hipMemset");
 
     // particle buffers
-    double factor = 2.0;
-    if (const char* env = getenv("QSG_CAPACITY_FACTOR")) factor = atof(env);
-    if (!(factor >= 1.0)) factor = 2.0;
-    double capD = factor * (double)T.nParticles + (double)T.sourcePerCycle + 65536.0;
-    if (capD > 4.0e9) throw QsgError(QSG_ERR_CAPACITY, "particle capacity exceeds 32-bit indexing");
-    uint32_t cap = (uint32_t)capD;
+    uint32_t cap = qsgi::particleCapacity(T);
     p->proc[0] = makeParticles(p, cap);
     p->proc[1] = makeParticles(p, cap);
     p->pop = makeParticles(p, cap);
@@ -424,17 +341,36 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
     p->phase = 0;
 }
 
-int fail(const std::exception& e) {
-    g_lastError = e.what();
-    if (const QsgError* q = dynamic_cast<const QsgError*>(&e)) return q->code;
-    return QSG_ERR_ARG;
+qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
+    return qsgi::createFromTables(params, qs::buildProblem(params), device);
 }
 
-qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
+}  // namespace
+
+uint32_t qsgi::particleCapacity(const qs::ProblemTables& T) {
+    double factor = 2.0;
+    if (const char* env = getenv("QSG_CAPACITY_FACTOR")) factor = atof(env);
+    if (!(factor >= 1.0)) factor = 2.0;
+    double capD = factor * (double)T.nParticles + (double)T.sourcePerCycle + 65536.0;
+    if (capD > 4.0e9) throw QsgError(QSG_ERR_CAPACITY, "particle capacity exceeds 32-bit indexing");
+    return (uint32_t)capD;
+}
+
+uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T) {
+    const uint64_t nc = (uint64_t)T.nCells, G = (uint64_t)T.nGroups, M = T.materials.size();
+    const uint64_t R = (uint64_t)T.maxReactions;
+    uint64_t b = nc * 96 * 8 + nc * 4 + nc * 8 + nc * 24 + (nc + 1) * 8 + nc * G * 8;
+    b += (G + 1) * 8 + M * G * 8 + M * R * G * 8 + M * R * 4;
+    const uint64_t cap = particleCapacity(T);
+    b += 5 * (cap * qs::kParticleBytes + 12 * 256);
+    return b + 64 * 1024;
+}
+
+qsg_problem_s* qsgi::createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device) {
     qsg_problem_s* p = new qsg_problem_s();
     try {
         p->params = params;
-        p->tables = qs::buildProblem(params);
+        p->tables = std::move(tables);
         build(p, device);
     } catch (...) {
         destroyProblem(p);
@@ -443,18 +379,7 @@ qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
     return p;
 }
 
-#define QSG_GUARD(...)                   \
-    try {                                \
-        __VA_ARGS__;                     \
-        return QSG_OK;                   \
-    } catch (const std::exception& e) {  \
-        return fail(e);                  \
-    } catch (...) {                      \
-        g_lastError = "unknown error";   \
-        return QSG_ERR_ARG;              \
-    }
-
-}  // namespace
+void qsgi::destroy(qsg_problem_s* p) { destroyProblem(p); }
 
 extern "C" {
 

@@ -1,0 +1,320 @@
+// Batched form of the C ABI: many independent problems advance one cycle
+// per step through shared kernel launches (device source, population
+// control, tracking passes, flux reduction).  Each problem keeps its own
+// buffers, tallies and RNG streams; the batch only shares launches, so a
+// problem's results are the same alone or in any mix.
+#include <cstring>
+#include <vector>
+
+#include "qsg_internal.h"
+
+using qsgi::hipCheck;
+using qsgi::QsgError;
+
+struct qsg_batch_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int numCUs = 0;
+    std::vector<qsg_problem_s*> probs;
+    qs::BatchSlot* dSlots = nullptr;
+    qs::BatchState* dState = nullptr;
+    qs::DevCounters* dCounters = nullptr;
+    uint64_t* dSrcPrefix = nullptr;
+    uint64_t* dPopPrefix = nullptr;
+    double* dPartials = nullptr;
+    double* dFluxSums = nullptr;
+    int partsPerProblem = 16;
+    qs::BatchState* hState = nullptr;  // pinned mirrors
+    qs::DevCounters* hCounters = nullptr;
+    double* hFluxSums = nullptr;
+    std::vector<uint32_t> active;
+    std::vector<int> expectedCycle;  // each problem's cycle as the batch left it
+    qs::TrackTuning tune = {256, 16, 16, 64};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> passEv;
+    qsg_stats stats = {};
+};
+
+namespace {
+
+template <typename T>
+T* devAlloc(size_t n) {
+    void* p = nullptr;
+    hipCheck(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+T* hostAlloc(size_t n) {
+    void* p = nullptr;
+    hipCheck(hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<T*>(p);
+}
+
+void destroyBatch(qsg_batch_s* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* d : {(void*)b->dSlots, (void*)b->dState, (void*)b->dCounters, (void*)b->dSrcPrefix,
+                    (void*)b->dPopPrefix, (void*)b->dPartials, (void*)b->dFluxSums})
+        if (d) (void)hipFree(d);
+    for (void* h : {(void*)b->hState, (void*)b->hCounters, (void*)b->hFluxSums})
+        if (h) (void)hipHostFree(h);
+    for (hipEvent_t e : {b->ev0, b->ev1})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->passEv) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+qsg_batch_s* createBatch(qsg_problem* problems, int n) {
+    if (n <= 0 || !problems) throw QsgError(QSG_ERR_ARG, "a batch needs at least one problem");
+    qsg_batch_s* b = new qsg_batch_s();
+    try {
+        b->device = problems[0]->device;
+        for (int i = 0; i < n; i++) {
+            qsg_problem_s* p = problems[i];
+            if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem in batch");
+            if (p->device != b->device) throw QsgError(QSG_ERR_ARG, "batch problems must share one device");
+            if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "batch problems must be at a cycle boundary");
+            if (p->cur != (p->cycle & 1))
+                throw QsgError(QSG_ERR_STATE, "problem buffer parity does not match its cycle");
+            b->probs.push_back(p);
+        }
+        hipCheck(hipSetDevice(b->device), "hipSetDevice");
+        hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipCheck(hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, b->device),
+                 "hipDeviceGetAttribute");
+        hipCheck(hipEventCreate(&b->ev0), "hipEventCreate");
+        hipCheck(hipEventCreate(&b->ev1), "hipEventCreate");
+        b->tune = problems[0]->tune;
+        b->dSlots = devAlloc<qs::BatchSlot>(n);
+        b->dState = devAlloc<qs::BatchState>(n);
+        b->dCounters = devAlloc<qs::DevCounters>(n);
+        b->dSrcPrefix = devAlloc<uint64_t>(n + 1);
+        b->dPopPrefix = devAlloc<uint64_t>(n + 1);
+        b->dPartials = devAlloc<double>((size_t)n * b->partsPerProblem);
+        b->dFluxSums = devAlloc<double>(n);
+        b->hState = hostAlloc<qs::BatchState>(n);
+        b->hCounters = hostAlloc<qs::DevCounters>(n);
+        b->hFluxSums = hostAlloc<double>(n);
+        std::vector<qs::BatchSlot> slots(n);
+        std::vector<qs::BatchState> st(n);
+        for (int i = 0; i < n; i++) {
+            qsg_problem_s* p = b->probs[i];
+            qs::BatchSlot& s = slots[i];
+            memset(&s, 0, sizeof(s));
+            s.prob = p->dp;
+            s.proc[0] = p->proc[0].v;
+            s.proc[1] = p->proc[1].v;
+            s.pop = p->pop.v;
+            s.extra[0] = p->extra[0].v;
+            s.extra[1] = p->extra[1].v;
+            s.cap = p->pop.cap;
+            s.loadBalance = (uint32_t)p->tables.loadBalance;
+            s.sourcePerCycle = p->tables.sourcePerCycle;
+            s.nParticles = p->tables.nParticles;
+            s.st = b->dState + i;
+            s.dc = b->dCounters + i;
+            memset(&st[i], 0, sizeof(st[i]));
+            st[i].nCensus = (uint32_t)p->startCount;
+            st[i].cycle = (uint32_t)p->cycle;
+            st[i].active = 1;
+        }
+        b->active.assign(n, 1);
+        for (int i = 0; i < n; i++) b->expectedCycle.push_back(b->probs[i]->cycle);
+        hipCheck(hipMemcpy(b->dSlots, slots.data(), n * sizeof(qs::BatchSlot), hipMemcpyHostToDevice), "hipMemcpy");
+        hipCheck(hipMemcpy(b->dState, st.data(), n * sizeof(qs::BatchState), hipMemcpyHostToDevice), "hipMemcpy");
+        hipCheck(hipMemset(b->dCounters, 0, n * sizeof(qs::DevCounters)), "hipMemset");
+        hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    } catch (...) {
+        destroyBatch(b);
+        throw;
+    }
+    return b;
+}
+
+void readStates(qsg_batch_s* b) {
+    const size_t n = b->probs.size();
+    hipCheck(hipMemcpyAsync(b->hState, b->dState, n * sizeof(qs::BatchState), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+void checkBatchErrors(qsg_batch_s* b) {
+    for (size_t i = 0; i < b->probs.size(); i++) {
+        uint32_t e = b->hState[i].error;
+        if (!e) continue;
+        std::string m = "problem " + std::to_string(i) + ": device error flags:";
+        if (e & 1u) m += " census buffer overflow;";
+        if (e & 2u) m += " secondary buffer overflow;";
+        if (e & 4u) m += " population-control buffer overflow;";
+        if (e & 8u) m += " runaway particle history;";
+        if (e & 16u) m += " fission produced more than 4 particles;";
+        throw QsgError((e & 7u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC, m);
+    }
+}
+
+// One cycle of every active problem (cycleInit + cycleTracking +
+// cycleFinalize of main.cc:55-66, for all of them at once).
+void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
+    hipCheck(hipSetDevice(b->device), "hipSetDevice");
+    const int n = (int)b->probs.size();
+    for (int i = 0; i < n; i++)
+        if (b->probs[i]->phase != 0 || b->probs[i]->cycle != b->expectedCycle[i])
+            throw QsgError(QSG_ERR_STATE, "a batch problem was advanced outside the batch");
+    qs::LaunchCtx c{(void*)b->stream, b->numCUs};
+    hipCheck(hipEventRecord(b->ev0, b->stream), "hipEventRecord");
+    qs::launchBatchPrep(c, b->dSlots, n, b->dSrcPrefix, b->dPopPrefix);
+    qs::launchBatchSource(c, b->dSlots, n, b->dSrcPrefix);
+    qs::launchBatchPopulation(c, b->dSlots, n, b->dPopPrefix);
+    // tracking passes until no problem has secondaries left
+    int k = 0;
+    double trackSeconds = 0;
+    for (;;) {
+        const int kind = (k == 0) ? 0 : ((k % 2 == 1) ? 1 : 2);
+        while ((int)b->passEv.size() < 2 * (k + 1)) {
+            hipEvent_t e;
+            hipCheck(hipEventCreate(&e), "hipEventCreate");
+            b->passEv.push_back(e);
+        }
+        qs::launchBatchPassPrep(c, b->dSlots, n, kind);
+        hipCheck(hipEventRecord(b->passEv[2 * k], b->stream), "hipEventRecord");
+        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune);
+        hipCheck(hipGetLastError(), "batch track launch");
+        hipCheck(hipEventRecord(b->passEv[2 * k + 1], b->stream), "hipEventRecord");
+        readStates(b);
+        checkBatchErrors(b);
+        const int outIdx = (kind == 1) ? 1 : 0;
+        uint64_t pending = 0;
+        for (int i = 0; i < n; i++)
+            if (b->hState[i].active) pending += b->hState[i].extra[outIdx];
+        k++;
+        if (pending == 0) break;
+    }
+    for (int q = 0; q < k; q++) {
+        float ms = 0;
+        hipCheck(hipEventElapsedTime(&ms, b->passEv[2 * q], b->passEv[2 * q + 1]), "hipEventElapsedTime");
+        trackSeconds += ms * 1e-3;
+    }
+    qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
+    hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(b->hCounters, b->dCounters, n * sizeof(qs::DevCounters), hipMemcpyDeviceToHost,
+                            b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipEventRecord(b->ev1, b->stream), "hipEventRecord");
+    readStates(b);
+    checkBatchErrors(b);
+    float stepMs = 0;
+    hipCheck(hipEventElapsedTime(&stepMs, b->ev0, b->ev1), "hipEventElapsedTime");
+
+    for (int i = 0; i < n; i++) {
+        const qs::BatchState& st = b->hState[i];
+        qsg_problem_s* p = b->probs[i];
+        if (!st.active) {
+            if (balances) memset(&balances[i], 0, sizeof(qsg_balance));
+            if (fluxSums) fluxSums[i] = 0.0;
+            continue;
+        }
+        const qs::DevCounters& d = b->hCounters[i];
+        if (balances) {
+            qsg_balance& bal = balances[i];
+            bal.start = st.start;
+            bal.source = p->tables.sourcePerCycle;
+            bal.rr = d.rr;
+            bal.split = d.split;
+            bal.absorb = d.absorb;
+            bal.scatter = d.scatter;
+            bal.fission = d.fission;
+            bal.produce = d.produce;
+            bal.collision = d.collision;
+            bal.escape = d.escape;
+            bal.census = d.census;
+            bal.num_segments = d.numSegments;
+            bal.end = st.nCensus;
+        }
+        if (fluxSums) fluxSums[i] = b->hFluxSums[i];
+        // the problem handle continues from the batch's state
+        p->startCount = st.nCensus;
+        p->nCensus = st.nCensus;
+        p->cycle = (int)st.cycle;
+        b->expectedCycle[i] = p->cycle;
+        p->cur = p->cycle & 1;
+        p->censusBuf = p->cur;
+        p->stats.segments += d.numSegments;
+        p->stats.collisions += d.collision;
+        p->stats.facet_crossings += d.numSegments - d.collision - d.census;
+        p->stats.particle_loads += st.start + p->tables.sourcePerCycle + d.split - d.rr + d.produce;
+        p->stats.cycles += 1;
+        b->stats.segments += d.numSegments;
+        b->stats.collisions += d.collision;
+        b->stats.facet_crossings += d.numSegments - d.collision - d.census;
+        b->stats.particle_loads += st.start + p->tables.sourcePerCycle + d.split - d.rr + d.produce;
+    }
+    b->stats.track_kernel_seconds += trackSeconds;
+    b->stats.track_launches += (uint64_t)k;
+    b->stats.tracking_seconds += stepMs * 1e-3;
+    b->stats.cycles += 1;
+}
+
+}  // namespace
+
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n) { return createBatch(problems, n); }
+void qsgiDestroyBatch(qsg_batch_s* b) { destroyBatch(b); }
+void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) { batchStep(b, balances, fluxSums); }
+void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t) { b->tune = t; }
+const qsg_stats& qsgiBatchStats(const qsg_batch_s* b) { return b->stats; }
+
+extern "C" {
+
+int qsg_batch_create(qsg_problem* problems, int n, qsg_batch* out) {
+    QSG_GUARD({
+        if (!out) throw QsgError(QSG_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        *out = createBatch(problems, n);
+    })
+}
+
+int qsg_batch_destroy(qsg_batch b) {
+    QSG_GUARD({ destroyBatch(b); })
+}
+
+int qsg_batch_step(qsg_batch b, qsg_balance* balances, double* flux_sums) {
+    QSG_GUARD({
+        if (!b) throw QsgError(QSG_ERR_ARG, "NULL batch");
+        batchStep(b, balances, flux_sums);
+    })
+}
+
+int qsg_batch_set_active(qsg_batch b, int index, int active) {
+    QSG_GUARD({
+        if (!b || index < 0 || index >= (int)b->probs.size()) throw QsgError(QSG_ERR_ARG, "bad batch index");
+        hipCheck(hipSetDevice(b->device), "hipSetDevice");
+        uint32_t a = active ? 1u : 0u;
+        hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        hipCheck(hipMemcpy(&b->dState[index].active, &a, sizeof(a), hipMemcpyHostToDevice), "hipMemcpy");
+        b->active[index] = a;
+    })
+}
+
+int qsg_batch_set_tuning(qsg_batch b, const qsg_tuning* t) {
+    QSG_GUARD({
+        if (!b || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (t->block_size < 64 || t->block_size > 256 || t->block_size % 64)
+            throw QsgError(QSG_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+        if (t->waves_per_cu < 1 || t->waves_per_cu > 64) throw QsgError(QSG_ERR_ARG, "waves_per_cu out of range");
+        if (t->refill_min < 1 || t->refill_min > 64) throw QsgError(QSG_ERR_ARG, "refill_min out of range");
+        if (t->chunk < 1 || t->chunk > 4096) throw QsgError(QSG_ERR_ARG, "chunk out of range");
+        b->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+    })
+}
+
+int qsg_batch_get_stats(qsg_batch b, qsg_stats* stats) {
+    QSG_GUARD({
+        if (!b || !stats) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *stats = b->stats;
+    })
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internals shared by the C-ABI translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qs_device.h"
+#include "qs_params.h"
+#include "qs_problem.h"
+#include "qsg.h"
+
+namespace qsgi {
+
+struct QsgError : std::runtime_error {
+    int code;
+    QsgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw QsgError(QSG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct PBuf {
+    void* mem = nullptr;
+    qs::Particles v = {};
+    uint32_t cap = 0;
+};
+
+// Records an error message for qsg_last_error() and returns its code.
+int fail(const std::exception& e);
+
+}  // namespace qsgi
+
+struct qsg_problem_s {
+    using PBuf = qsgi::PBuf;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int numCUs = 0;
+    qs::Parameters params;
+    qs::ProblemTables tables;
+    qs::DevProblem dp = {};
+    std::vector<void*> allocs;
+    size_t deviceBytes = 0;
+
+    PBuf proc[2], pop, extra[2];
+    int cur = 0;               // proc[cur] holds this cycle's processing population
+    uint64_t startCount = 0;   // census particles carried into this cycle
+    uint32_t nPop = 0;         // processing population after population control
+    uint32_t nCensus = 0;      // census count after tracking
+    int censusBuf = -1;        // proc index holding the last census
+    int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
+    int cycle = 0;
+
+    qs::PassCounters* pc = nullptr;
+    qs::DevCounters* dc = nullptr;
+    qs::PassCounters* hpc = nullptr;  // pinned host mirrors
+    qs::DevCounters* hdc = nullptr;
+    double* partials = nullptr;
+    int nPartials = 0;
+    double* fluxSumDev = nullptr;
+    double* hFluxSum = nullptr;
+
+    qs::TrackTuning tune = {256, 16, 16, 64};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evI0 = nullptr, evI1 = nullptr, evF0 = nullptr, evF1 = nullptr;
+    std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
+    double lastTrackingSeconds = 0.0;
+    qsg_stats stats = {};
+
+    qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
+
+    template <typename T>
+    T* dalloc(size_t count) {
+        void* p = nullptr;
+        size_t bytes = count * sizeof(T);
+        if (bytes == 0) bytes = 8;
+        qsgi::hipCheck(hipMalloc(&p, bytes), "hipMalloc");
+        allocs.push_back(p);
+        deviceBytes += bytes;
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& h) {
+        T* d = dalloc<T>(h.size());
+        if (!h.empty())
+            qsgi::hipCheck(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+        return d;
+    }
+};
+
+namespace qsgi {
+// Particle-buffer capacity and total device bytes a problem will allocate.
+uint32_t particleCapacity(const qs::ProblemTables& T);
+uint64_t deviceBytesEstimate(const qs::ProblemTables& T);
+// Uploads already-built tables and allocates the device state.
+qsg_problem_s* createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device);
+void destroy(qsg_problem_s* p);
+}  // namespace qsgi
+
+// Batch internals used by the serving loop.
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n);
+void qsgiDestroyBatch(qsg_batch_s* b);
+void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums);
+void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t);
+const qsg_stats& qsgiBatchStats(const qsg_batch_s* b);
+
+#define QSG_GUARD(...)                   \
+    try {                                \
+        __VA_ARGS__;                     \
+        return QSG_OK;                   \
+    } catch (const std::exception& e) {  \
+        return qsgi::fail(e);            \
+    } catch (...) {                      \
+        return qsgi::fail(std::runtime_error("unknown error")); \
+    }

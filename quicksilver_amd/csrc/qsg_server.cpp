@@ -1,0 +1,266 @@
+// Serving loop over the batched form: solve requests (input decks) are
+// queued, admitted in arrival order while their device footprint fits the
+// HBM budget, kept resident on the device across cycles, advanced together
+// one cycle per qsg_server_step, and retired (device memory released) when
+// they have run their nSteps cycles.  Continuous batching: the batch is
+// rebuilt at a cycle boundary whenever a request joins or leaves, so work
+// arriving while others run shares their launches.  No reference
+// counterpart: the reference runs one problem per process.
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "qsg_internal.h"
+
+using qsgi::hipCheck;
+using qsgi::QsgError;
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    qs::Parameters params;
+    qs::ProblemTables tables;
+    uint64_t bytes = 0;
+    int nSteps = 0;
+    int state = 0;  // 0 queued, 1 active, 2 done, 3 failed
+    qsg_problem_s* prob = nullptr;
+    std::vector<qsg_balance> bal;
+    std::vector<double> flux;
+    std::string error;
+};
+
+}  // namespace
+
+struct qsg_server_s {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int maxActive = 0;
+    uint64_t nextId = 1;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    std::map<uint64_t, std::unique_ptr<Request>> all;
+    qsg_batch_s* batch = nullptr;
+    bool dirty = true;  // membership changed since the batch was built
+    qs::TrackTuning tune = {256, 16, 16, 64};
+    bool haveTune = false;
+    qsg_server_stats stats = {};
+};
+
+namespace {
+
+void dropBatch(qsg_server_s* s) {
+    if (s->batch) qsgiDestroyBatch(s->batch);
+    s->batch = nullptr;
+}
+
+void retire(qsg_server_s* s, Request* r, int state) {
+    if (r->prob) {
+        qsgi::destroy(r->prob);
+        r->prob = nullptr;
+        s->used -= r->bytes;
+    }
+    r->state = state;
+    s->dirty = true;
+}
+
+// Admit queued requests in arrival order while they fit the budget (and the
+// active-count cap); head-of-line requests that never fit are failed.
+void admit(qsg_server_s* s) {
+    while (!s->queue.empty()) {
+        Request* r = s->queue.front();
+        if (r->bytes > s->budget) {
+            s->queue.pop_front();
+            r->error = "request needs more device memory than the whole budget";
+            r->state = 3;
+            s->stats.rejected++;
+            continue;
+        }
+        if (s->used + r->bytes > s->budget) break;
+        if (s->maxActive > 0 && (int)s->active.size() >= s->maxActive) break;
+        s->queue.pop_front();
+        try {
+            r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device);
+            r->tables = qs::ProblemTables();
+            s->used += r->bytes;
+            r->state = 1;
+            s->active.push_back(r);
+            s->stats.admitted++;
+            s->dirty = true;
+        } catch (const std::exception& e) {
+            r->error = e.what();
+            r->state = 3;
+            s->stats.rejected++;
+        }
+    }
+}
+
+void serverStep(qsg_server_s* s) {
+    admit(s);
+    if (s->active.empty()) return;
+    if (s->dirty || !s->batch) {
+        dropBatch(s);
+        std::vector<qsg_problem> hs;
+        for (Request* r : s->active) hs.push_back(r->prob);
+        s->batch = qsgiCreateBatch(hs.data(), (int)hs.size());
+        if (s->haveTune) qsgiBatchSetTuning(s->batch, s->tune);
+        s->dirty = false;
+    }
+    const size_t n = s->active.size();
+    std::vector<qsg_balance> bal(n);
+    std::vector<double> flux(n);
+    try {
+        qsgiBatchStep(s->batch, bal.data(), flux.data());
+    } catch (...) {
+        // a failed step leaves the problems' device state undefined
+        for (Request* r : s->active) {
+            r->error = "batch step failed";
+            retire(s, r, 3);
+        }
+        s->active.clear();
+        dropBatch(s);
+        throw;
+    }
+    s->stats.steps++;
+    std::vector<Request*> still;
+    for (size_t i = 0; i < n; i++) {
+        Request* r = s->active[i];
+        r->bal.push_back(bal[i]);
+        r->flux.push_back(flux[i]);
+        s->stats.segments += bal[i].num_segments;
+        if ((int)r->bal.size() >= r->nSteps) {
+            retire(s, r, 2);
+            s->stats.completed++;
+        } else {
+            still.push_back(r);
+        }
+    }
+    s->active.swap(still);
+    if (s->dirty) dropBatch(s);
+}
+
+Request* find(qsg_server_s* s, uint64_t id) {
+    auto it = s->all.find(id);
+    if (it == s->all.end()) throw QsgError(QSG_ERR_ARG, "unknown request id");
+    return it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsg_server_create(int device, uint64_t hbm_budget_bytes, int max_active, qsg_server* out) {
+    QSG_GUARD({
+        if (!out) throw QsgError(QSG_ERR_ARG, "out is NULL");
+        if (hbm_budget_bytes == 0) throw QsgError(QSG_ERR_ARG, "budget must be positive");
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        qsg_server_s* s = new qsg_server_s();
+        s->device = device;
+        s->budget = hbm_budget_bytes;
+        s->maxActive = max_active;
+        *out = s;
+    })
+}
+
+int qsg_server_destroy(qsg_server s) {
+    QSG_GUARD({
+        if (!s) return QSG_OK;
+        dropBatch(s);
+        for (Request* r : s->active) retire(s, r, 3);
+        delete s;
+    })
+}
+
+int qsg_server_submit(qsg_server s, const char* deck_text, uint64_t* request_id) {
+    QSG_GUARD({
+        if (!s || !deck_text || !request_id) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        std::unique_ptr<Request> r(new Request());
+        const char* argv0[] = {"qs"};
+        r->params = qs::getParameters(1, argv0);
+        r->params.geometryParams.clear();
+        r->params.materialParams.clear();
+        r->params.crossSectionParams.clear();
+        qs::parseDeckText(deck_text, r->params);
+        qs::supplyDefaults(r->params);
+        r->tables = qs::buildProblem(r->params);
+        r->bytes = qsgi::deviceBytesEstimate(r->tables);
+        r->nSteps = r->tables.nSteps;
+        r->id = s->nextId++;
+        *request_id = r->id;
+        s->queue.push_back(r.get());
+        s->all[r->id] = std::move(r);
+        s->stats.submitted++;
+    })
+}
+
+int qsg_server_step(qsg_server s, int* n_active, int* n_queued) {
+    QSG_GUARD({
+        if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
+        serverStep(s);
+        admit(s);
+        if (n_active) *n_active = (int)s->active.size();
+        if (n_queued) *n_queued = (int)s->queue.size();
+    })
+}
+
+int qsg_server_status(qsg_server s, uint64_t id, int* state, int* cycles_done) {
+    QSG_GUARD({
+        if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
+        Request* r = find(s, id);
+        if (state) *state = r->state;
+        if (cycles_done) *cycles_done = (int)r->bal.size();
+    })
+}
+
+int qsg_server_result(qsg_server s, uint64_t id, qsg_balance* balances, double* flux_sums, int n) {
+    QSG_GUARD({
+        if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
+        Request* r = find(s, id);
+        if (r->state == 3) throw QsgError(QSG_ERR_STATE, "request failed: " + r->error);
+        if (n > (int)r->bal.size()) throw QsgError(QSG_ERR_ARG, "more cycles requested than completed");
+        for (int i = 0; i < n; i++) {
+            if (balances) balances[i] = r->bal[i];
+            if (flux_sums) flux_sums[i] = r->flux[i];
+        }
+    })
+}
+
+int qsg_server_release(qsg_server s, uint64_t id) {
+    QSG_GUARD({
+        if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
+        Request* r = find(s, id);
+        if (r->state == 1) throw QsgError(QSG_ERR_STATE, "request is still running");
+        if (r->state == 0) {
+            for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+                if (*it == r) {
+                    s->queue.erase(it);
+                    break;
+                }
+        }
+        s->all.erase(id);
+    })
+}
+
+int qsg_server_set_tuning(qsg_server s, const qsg_tuning* t) {
+    QSG_GUARD({
+        if (!s || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        s->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+        s->haveTune = true;
+        if (s->batch) qsgiBatchSetTuning(s->batch, s->tune);
+    })
+}
+
+int qsg_server_get_stats(qsg_server s, qsg_server_stats* stats) {
+    QSG_GUARD({
+        if (!s || !stats) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *stats = s->stats;
+        stats->bytes_in_use = s->used;
+        stats->active = (uint64_t)s->active.size();
+        stats->queued = (uint64_t)s->queue.size();
+    })
+}
+
+}  // extern "C"

@@ -247,9 +247,13 @@ def _shape(c) -> np.ndarray:
     return v / v[_NORM_G]
 
 
-def _material_xs(rng: np.random.Generator, name: str, n_iso: int, lo: float = 0.1, hi: float = 10.0):
+def _material_xs(rng: np.random.Generator, name: str, n_iso: int, lo: float = 0.1, hi: float = 10.0,
+                 k_max: float = 0.0):
     """Draw one material's three reaction polynomials, ratios, total and nuBar,
-    rejecting draws whose per-group macroscopic total leaves [lo, hi]."""
+    rejecting draws whose per-group macroscopic total leaves [lo, hi] and,
+    when k_max > 0, draws whose particle multiplication per absorbing
+    collision, nuBar * sigma_f / (sigma_a + sigma_f), exceeds k_max in any
+    group (so a population cannot grow without bound within a time step)."""
     for _ in range(10000):
         coeffs = {}
         for kind in ("scatter", "absorb", "fission"):
@@ -264,6 +268,11 @@ def _material_xs(rng: np.random.Generator, name: str, n_iso: int, lo: float = 0.
         nubar = float(rng.uniform(1.5, 2.5))
         wsum = sum(ratios.values())
         tot = sum(total * ratios[k] / wsum * _shape(coeffs[k]) for k in coeffs)
+        if k_max > 0:
+            sa = ratios["absorb"] * _shape(coeffs["absorb"])
+            sf = ratios["fission"] * _shape(coeffs["fission"])
+            if np.max(nubar * sf / np.maximum(sa + sf, 1e-300)) > k_max:
+                continue
         if tot.min() >= lo and tot.max() <= hi:
             break
     else:  # pragma: no cover
@@ -340,7 +349,7 @@ def _random_problem(rng: np.random.Generator, n_steps: int, tag: str) -> Deck:
     mats, xss, geo = [], [], []
     names = ["matA", "matB", "matC"][:n_mat]
     for idx, name in enumerate(names):
-        m, x = _material_xs(rng, name, int(rng.integers(5, 21)))
+        m, x = _material_xs(rng, name, int(rng.integers(5, 21)), k_max=1.0)
         mats.append(m)
         xss += x
         lo = n * idx / (2.0 * n_mat + 1)

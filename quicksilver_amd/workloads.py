@@ -1,0 +1,116 @@
+"""Drivers for the batched and serving workloads (used by bench.py).
+
+* ``run_one_at_a_time(decks)`` -- each deck alone on the GPU, in order
+  (the plain method, one problem per launch sequence).
+* ``run_batched(decks, tuning)`` -- all decks in one qsg_batch: one set of
+  launches per cycle for every problem.
+* ``serve_stream(requests, ...)`` -- Poisson arrivals served by the
+  continuous-batching qsg_server (or, for the baseline, one at a time in
+  arrival order); reports aggregate segments/s and p50/p99 latency.
+"""
+from __future__ import annotations
+
+import time
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+import quicksilver_amd as q
+from quicksilver_amd.decks import Deck, Request
+
+
+def run_one_at_a_time(decks: Sequence[Deck], tuning: Optional[Tuple[int, int, int, int]] = None) -> Dict:
+    t0 = time.perf_counter()
+    segments = 0
+    per = []
+    for d in decks:
+        with q.Problem(d.text()) as p:
+            if tuning:
+                p.set_tuning(*tuning)
+            out = p.run()
+        per.append([b for b, _, _ in out])
+        segments += sum(b["num_segments"] for b, _, _ in out)
+    wall = time.perf_counter() - t0
+    return {"segments": segments, "wall_s": wall, "segments_per_s": segments / wall, "balances": per}
+
+
+def run_batched(decks: Sequence[Deck], tuning: Optional[Tuple[int, int, int, int]] = None) -> Dict:
+    t0 = time.perf_counter()
+    probs = [q.Problem(d.text()) for d in decks]
+    t1 = time.perf_counter()
+    try:
+        with q.Batch(probs) as b:
+            if tuning:
+                b.set_tuning(*tuning)
+            out = b.run([int(d.simulation["nSteps"]) for d in decks])
+            st = b.stats()
+    finally:
+        for p in probs:
+            p.close()
+    wall = time.perf_counter() - t0
+    segments = sum(bal["num_segments"] for per in out for bal, _ in per)
+    return {"segments": segments, "wall_s": wall, "setup_s": t1 - t0, "segments_per_s": segments / wall,
+            "balances": [[bal for bal, _ in per] for per in out], "stats": st}
+
+
+def _percentiles(lat: List[float]) -> Dict[str, float]:
+    a = np.array(lat)
+    return {"p50_s": float(np.percentile(a, 50)), "p99_s": float(np.percentile(a, 99)),
+            "mean_s": float(a.mean()), "max_s": float(a.max())}
+
+
+def serve_stream(requests: Sequence[Request], batched: bool = True, hbm_budget_bytes: int = 200 << 30,
+                 tuning: Optional[Tuple[int, int, int, int]] = None) -> Dict:
+    """Replays the arrival times in real time.  Latency = completion time -
+    arrival time of each request."""
+    arrivals = [r.arrival_s for r in requests]
+    t0 = time.perf_counter()
+    lat: List[float] = []
+    segments = 0
+    if not batched:
+        clock = 0.0
+        for r in requests:
+            now = time.perf_counter() - t0
+            if now < r.arrival_s:
+                time.sleep(r.arrival_s - now)
+            with q.Problem(r.deck.text()) as p:
+                if tuning:
+                    p.set_tuning(*tuning)
+                out = p.run()
+            segments += sum(b["num_segments"] for b, _, _ in out)
+            lat.append(time.perf_counter() - t0 - r.arrival_s)
+        wall = time.perf_counter() - t0
+        return {"requests": len(requests), "segments": segments, "wall_s": wall,
+                "segments_per_s": segments / wall, **_percentiles(lat), "mode": "one-at-a-time"}
+    with q.Server(hbm_budget_bytes) as s:
+        if tuning:
+            s.set_tuning(*tuning)
+        pending = list(range(len(requests)))
+        running: Dict[int, int] = {}
+        nxt = 0
+        done = 0
+        while done < len(requests):
+            now = time.perf_counter() - t0
+            while nxt < len(requests) and arrivals[nxt] <= now:
+                running[s.submit(requests[nxt].deck.text())] = nxt
+                nxt += 1
+            if not running:
+                time.sleep(max(0.0, arrivals[nxt] - now))
+                continue
+            s.step()
+            now = time.perf_counter() - t0
+            for rid, idx in list(running.items()):
+                st, _ = s.status(rid)
+                if st in (q.Server.DONE, q.Server.FAILED):
+                    if st == q.Server.FAILED:
+                        raise RuntimeError(f"request {idx} failed")
+                    segments += sum(b["num_segments"] for b, _ in s.result(rid))
+                    lat.append(now - arrivals[idx])
+                    s.release(rid)
+                    del running[rid]
+                    done += 1
+        stats = s.stats()
+    wall = time.perf_counter() - t0
+    del pending
+    return {"requests": len(requests), "segments": segments, "wall_s": wall, "segments_per_s": segments / wall,
+            **_percentiles(lat), "mode": "continuous batching", "server": stats}

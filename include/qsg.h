@@ -77,6 +77,12 @@ typedef struct qsg_tuning {
     int32_t waves_per_cu; /* persistent waves launched per compute unit    */
     int32_t refill_min;   /* idle lanes (1..64) that trigger a wave refill  */
     int32_t chunk;        /* queue entries a wave claims at a time          */
+    int32_t stack_entries; /* per-wave secondary stack entries, 0 or 512..65536;
+                              0 sends secondaries through extra passes      */
+    int32_t sort_interval; /* sort the population by cell every n cycles
+                              (0: never)                                     */
+    int32_t waves_per_simd; /* register budget of the tracking kernels: 2, 3
+                               or 4 resident waves per SIMD                  */
 } qsg_tuning;
 
 const char* qsg_last_error(void);

@@ -80,7 +80,24 @@ class ServerStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
-                ("refill_min", ctypes.c_int32), ("chunk", ctypes.c_int32)]
+                ("refill_min", ctypes.c_int32), ("chunk", ctypes.c_int32),
+                ("stack_entries", ctypes.c_int32), ("sort_interval", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+# Launch shape used when none is set (the library's own default).
+DEFAULT_TUNING = {"block_size": 256, "waves_per_cu": 16, "refill_min": 16, "chunk": 64,
+                  "stack_entries": 512, "sort_interval": 0, "waves_per_simd": 3}
+
+
+def _tuning(kw: dict) -> Tuning:
+    unknown = set(kw) - set(DEFAULT_TUNING)
+    if unknown:
+        raise TypeError(f"unknown tuning knobs: {sorted(unknown)}")
+    return Tuning(**{**DEFAULT_TUNING, **kw})
 
 
 _P = ctypes.POINTER
@@ -250,8 +267,9 @@ class Problem:
         _check(lib().qsg_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
-        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+    def set_tuning(self, **knobs):
+        """Launch shape; knobs not given take DEFAULT_TUNING's values."""
+        t = _tuning(knobs)
         _check(lib().qsg_set_tuning(self._h, ctypes.byref(t)))
 
     def stats(self) -> dict:
@@ -353,8 +371,9 @@ class Batch:
     def set_active(self, index: int, active: bool):
         _check(lib().qsg_batch_set_active(self._h, index, 1 if active else 0))
 
-    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
-        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+    def set_tuning(self, **knobs):
+        """Launch shape; knobs not given take DEFAULT_TUNING's values."""
+        t = _tuning(knobs)
         _check(lib().qsg_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def stats(self) -> dict:
@@ -431,8 +450,9 @@ class Server:
     def release(self, rid: int):
         _check(lib().qsg_server_release(self._h, rid))
 
-    def set_tuning(self, block_size: int, waves_per_cu: int, refill_min: int, chunk: int):
-        t = Tuning(block_size, waves_per_cu, refill_min, chunk)
+    def set_tuning(self, **knobs):
+        """Launch shape; knobs not given take DEFAULT_TUNING's values."""
+        t = _tuning(knobs)
         _check(lib().qsg_server_set_tuning(self._h, ctypes.byref(t)))
 
     def stats(self) -> dict:

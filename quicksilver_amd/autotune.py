@@ -1,11 +1,15 @@
 """Autotuner for the tracking-pass launch shape.
 
 Knobs (qsg_tuning):
-  block_size    threads per workgroup (64..256)
-  waves_per_cu  persistent waves launched per compute unit
-  refill_min    idle lanes that make a wave refill from its queue
-                (how full a wave is kept with particles)
-  chunk         queue entries a wave claims per atomic
+  block_size     threads per workgroup (64..256)
+  waves_per_cu   persistent waves launched per compute unit
+  refill_min     idle lanes that make a wave refill from its queue
+                 (how full a wave is kept with particles)
+  chunk          queue entries a wave claims per atomic
+  stack_entries  per-wave secondary stack (0: secondaries in extra passes)
+  sort_interval  sort the population by cell every n cycles (0: never)
+  waves_per_simd register budget of the tracking kernels (2, 3 or 4
+                 resident waves per SIMD: occupancy against spills)
 
 Method: coordinate descent from the current best.  Each knob in turn is swept
 over its candidate values with the others fixed, scoring every candidate by
@@ -33,16 +37,15 @@ SPACE = {
     "waves_per_cu": [4, 8, 12, 16, 24, 32],
     "refill_min": [1, 4, 8, 16, 32],
     "chunk": [16, 32, 64, 128, 256],
+    "stack_entries": [0, 512, 2048],
+    "sort_interval": [0, 1, 2, 4],
+    "waves_per_simd": [2, 3, 4],
 }
-ORDER = ["waves_per_cu", "refill_min", "chunk", "block_size"]
-DEFAULT = {"block_size": 256, "waves_per_cu": 16, "refill_min": 16, "chunk": 64}
+ORDER = ["waves_per_simd", "waves_per_cu", "refill_min", "chunk", "block_size", "stack_entries"]
+DEFAULT = dict(q.DEFAULT_TUNING)
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 Shape = Dict[str, int]
-
-
-def as_tuple(s: Shape) -> Tuple[int, int, int, int]:
-    return (s["block_size"], s["waves_per_cu"], s["refill_min"], s["chunk"])
 
 
 def signature(kind: str, decks: Sequence[str]) -> str:
@@ -70,7 +73,7 @@ def load_table(path: str = TABLE) -> Dict[str, Shape]:
 def lookup(sig: str, path: str = TABLE) -> Optional[Shape]:
     t = load_table(path)
     if sig in t:
-        return {k: int(v) for k, v in t[sig].items() if k in DEFAULT}
+        return {**DEFAULT, **{k: int(v) for k, v in t[sig].items() if k in DEFAULT}}
     return None
 
 
@@ -112,7 +115,7 @@ def tune_plain(deck_text: str, warmup: int = 2, cycles: int = 2, rounds: int = 2
         p.run(warmup)
 
         def score(s: Shape) -> float:
-            p.set_tuning(*as_tuple(s))
+            p.set_tuning(**s)
             p.reset_stats()
             p.run(cycles)
             st = p.stats()
@@ -130,7 +133,7 @@ def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 1, roun
                 b.step()
 
             def score(s: Shape) -> float:
-                b.set_tuning(*as_tuple(s))
+                b.set_tuning(**s)
                 st0 = b.stats()
                 for _ in range(cycles):
                     b.step()

@@ -106,7 +106,29 @@ struct TrackTuning {
     int wavesPerCU;    // persistent waves per CU
     int refillMin;     // idle lanes that trigger a refill
     int chunk;         // particles a wave claims per queue access
+    int stackCap;      // per-wave secondary stack entries (0: extra passes only)
+    int sortInterval;  // sort the population by cell every n cycles (0: never)
+    int wavesPerSimd;  // register budget of the tracking kernels: 2, 3 or 4 waves per SIMD
 };
+
+// Per-wave secondary stacks of a tracking launch.  Wave w (global wave
+// index) owns `cap` particle entries laid out like a particle buffer, at
+// base + w * 12 * cap * 8 with field stride cap * 8.  A wave pushes the
+// outputs of its multi-particle collisions there and pops them before it
+// takes new queue entries, so secondaries are tracked in the same launch
+// (depth first) instead of in a later pass.  Waves with index >= nWaves,
+// or cap == 0, send secondaries to the pass's extra queue instead.
+struct WaveStacks {
+    char* base;
+    uint32_t cap;
+    uint32_t nWaves;
+};
+constexpr uint32_t kStackCap = 512;
+// room that must be left on a stack before an iteration may push to it:
+// one collision per lane, at most 4 outputs each
+constexpr uint32_t kStackMargin = 256;
+inline size_t waveStackBytes(uint32_t nWaves, uint32_t cap) { return (size_t)nWaves * 12 * cap * 8; }
+constexpr TrackTuning kDefaultTuning = {256, 16, 16, 64, (int)kStackCap, 0, 3};
 
 // ---- batched form: many independent problems per launch ----
 // Per-problem mutable state of a batch (device memory).
@@ -160,7 +182,12 @@ void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, u
 // One tracking pass over `in` (count read from device memory *nIn; nInHost
 // only sizes the grid).  Census particles are appended to `census`
 // (pc->nCensus), secondaries to `extra` (*nExtraOut).
-void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune,
+// Blocks a tracking pass over nIn queued particles launches (persistent
+// grid of tune.wavesPerCU waves per CU, fewer for small queues).
+bool tablesFitLds(const DevProblem& P);
+long long trackPassBlocks(const LaunchCtx& ctx, const TrackTuning& tune, uint32_t nIn);
+long long batchTrackBlocks(const LaunchCtx& ctx, const TrackTuning& tune);
+void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune, const WaveStacks& ws,
                      Particles in, const uint32_t* nIn, uint32_t nInHost,
                      Particles census, uint32_t censusCap,
                      Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
@@ -178,7 +205,8 @@ void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint
 void launchBatchPopulation(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* popPrefix);
 // passKind 0: pop -> extra[0]; 1: extra[0] -> extra[1]; 2: extra[1] -> extra[0]
 void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind);
-void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune);
+void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
+                      const WaveStacks& ws);
 void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,
                          double* fluxSums);
 

@@ -458,6 +458,24 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t coun
     return base + prefix;
 }
 
+// wave_append onto a wave-private counter in LDS (no atomic needed: only
+// this wave's leader lane touches it).
+__device__ __forceinline__ uint32_t wave_push(uint32_t* top, uint32_t count) {
+    uint64_t act = __ballot(1);
+    uint64_t b0 = __ballot(count > 0), b1 = __ballot(count > 1), b2 = __ballot(count > 2),
+             b3 = __ballot(count > 3);
+    uint32_t total = __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
+    uint64_t lt = lanemask_lt();
+    uint32_t prefix = __popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt) + __popcll(b3 & lt);
+    int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == leader) {
+        base = *top;
+        *top = base + total;
+    }
+    base = __shfl(base, leader);
+    return base + prefix;
+}
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -527,8 +545,63 @@ struct SingleQueue {
     __device__ __forceinline__ const DevProblem& prob() const { return P; }
     __device__ __forceinline__ Tables tables() const { return T; }
     __device__ __forceinline__ QueueIO queue() const { return io; }
+    __device__ __forceinline__ void stage() const {}
     __device__ __forceinline__ bool next() { return false; }
 };
+
+// Wave-uniform copies of values read from device memory: the batch
+// kernel's problem descriptors are read through vector loads, and
+// readfirstlane moves the results into scalar registers (the compiler cannot
+// prove such a load uniform by itself), which keeps them out of the vector
+// register budget the tracking loop needs.
+__device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double rfld(double v) {
+    return __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, v)));
+}
+template <class T>
+__device__ __forceinline__ T* rflp(T* p) {
+    return (T*)rfl64((uint64_t)p);
+}
+__device__ __forceinline__ Particles rflv(const Particles& v) { return Particles{rflp(v.base), rfl64(v.stride)}; }
+
+__device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
+    DevProblem P;
+    P.nx = rfl(c.nx);
+    P.ny = rfl(c.ny);
+    P.nz = rfl(c.nz);
+    P.nCells = rfl(c.nCells);
+    P.dx = rfld(c.dx);
+    P.dy = rfld(c.dy);
+    P.dz = rfld(c.dz);
+    P.hdx = rfld(c.hdx);
+    P.hdy = rfld(c.hdy);
+    P.hdz = rfld(c.hdz);
+    P.boundaryMask = rfl(c.boundaryMask);
+    P.planes = rflp(c.planes);
+    P.cellMat = rflp(c.cellMat);
+    P.cellVolume = rflp(c.cellVolume);
+    P.cellCenter = rflp(c.cellCenter);
+    P.srcOffset = rflp(c.srcOffset);
+    P.G = rfl(c.G);
+    P.nMat = rfl(c.nMat);
+    P.maxR = rfl(c.maxR);
+    P.energies = rflp(c.energies);
+    P.total = rflp(c.total);
+    P.macro = rflp(c.macro);
+    P.reactType = rflp(c.reactType);
+    P.dt = rfld(c.dt);
+    P.eMin = rfld(c.eMin);
+    P.eMax = rfld(c.eMax);
+    P.sourceWeight = rfld(c.sourceWeight);
+    P.lowWeightCutoff = rfld(c.lowWeightCutoff);
+    P.flux = rflp(c.flux);
+    return P;
+}
 
 struct BatchQueue {
     static constexpr bool kMulti = true;
@@ -536,16 +609,28 @@ struct BatchQueue {
     int n;
     int passKind;
     int p;
-    __device__ __forceinline__ const BatchSlot& slot() const { return slots[p]; }
-    __device__ __forceinline__ const DevProblem& prob() const { return slots[p].prob; }
+    MatInfo* sMat;     // this wave's LDS copy of the current problem's materials
+    int32_t* sReact;   // ... and reaction types
+    __device__ __forceinline__ DevProblem prob() const { return load_problem(slots[p].prob); }
+    // Copies the current problem's material records and reaction types into
+    // the wave's LDS slots (read on every collision).
+    __device__ __forceinline__ void stage() const {
+        const DevProblem& P = slots[p].prob;
+        const int nM = rfl(P.nMat), nR = rfl(P.nMat * P.maxR);
+        for (int q = __lane_id(); q < nM; q += 64) sMat[q] = P.mat[q];
+        for (int q = __lane_id(); q < nR; q += 64) sReact[q] = P.reactType[q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     __device__ __forceinline__ Tables tables() const {
         const DevProblem& P = slots[p].prob;
         Tables T;
-        T.energies = P.energies;
-        T.total = P.total;
-        T.macro = P.macro;
-        T.reactType = P.reactType;
-        T.mat = P.mat;
+        T.energies = rflp(P.energies);
+        T.total = rflp(P.total);
+        T.macro = rflp(P.macro);
+        T.reactType = sReact;
+        T.mat = sMat;
         return T;
     }
     __device__ __forceinline__ uint32_t* inCount(int q) const {
@@ -554,19 +639,21 @@ struct BatchQueue {
     }
     __device__ __forceinline__ QueueIO queue() const {
         const BatchSlot& S = slots[p];
-        BatchState* st = S.st;
+        BatchState* st = rflp(S.st);
         const int outIdx = passKind == 1 ? 1 : 0;
-        QueueIO io = {passKind == 0 ? S.pop : S.extra[passKind == 1 ? 0 : 1],
-                      (uint32_t)__builtin_amdgcn_readfirstlane(*inCount(p)),
+        const uint32_t cyc = (uint32_t)rfl((int)st->cycle);
+        const uint32_t cap = (uint32_t)rfl((int)S.cap);
+        QueueIO io = {rflv(passKind == 0 ? S.pop : S.extra[passKind == 1 ? 0 : 1]),
+                      (uint32_t)rfl((int)*inCount(p)),
                       &st->head,
-                      S.proc[(st->cycle + 1) & 1],
+                      rflv(S.proc[(cyc + 1) & 1]),
                       &st->nCensus,
-                      S.cap,
-                      S.extra[outIdx],
+                      cap,
+                      rflv(S.extra[outIdx]),
                       &st->extra[outIdx],
-                      S.cap,
+                      cap,
                       &st->error,
-                      S.dc};
+                      rflp(S.dc)};
         return io;
     }
     // Move to the next problem (round robin) whose queue still has entries.
@@ -602,17 +689,27 @@ __device__ __forceinline__ void flush_tallies(DevCounters* dc, uint32_t& cSeg, u
     cSeg = cCol = cScat = cAbs = cFis = cProd = cEsc = cCen = 0;
 }
 
-// Tracks every particle of the wave's queue(s) to the end of its history
-// in this pass (CycleTrackingGuts, CycleTracking.cc:15-119).  Idle lanes are
-// refilled from the queue; when the queue is drained the wave moves on to the
-// accessor's next queue, flushing its tallies first.
-template <class Q>
-__device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
-    QueueIO io = acc.queue();
-    uint32_t nIn = io.nIn;
+// Per-wave secondary stack: a view into the launch's stack memory and its
+// top, kept in LDS so the lane that reserves entries inside divergent code
+// can publish the new top to the rest of the wave.
+struct StackRef {
+    Particles view;
+    uint32_t cap;
+    uint32_t* top;
+};
+
+// Tracks one problem's pass queue until it is drained (queue exhausted,
+// stack empty, every lane idle), then adds the wave's balance counters to
+// the problem's tallies.
+__device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T, const QueueIO& io,
+                                            const TrackTuning& tune, const StackRef& sr) {
+    const int G = P.G;
+    const uint32_t nIn = io.nIn;
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
-
+    const Particles stack = sr.view;
+    const uint32_t scap = sr.cap;
+    uint32_t* sTopW = sr.top;
     uint32_t qBase = 0, qEnd = 0;
     bool exhausted = false;
     bool active = false;
@@ -623,43 +720,53 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
 
     for (;;) {
-        const DevProblem& P = acc.prob();
-        const Tables T = acc.tables();
-        const int G = P.G;
-        // ---- refill idle lanes from the pass queue (wave-uniform control) ----
+        // ---- refill idle lanes: the wave's stack first (most recent
+        // secondaries), then the pass queue (wave-uniform control) ----
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t top = (uint32_t)__builtin_amdgcn_readfirstlane(*sTopW);
         uint64_t idle = __ballot(!active);
-        if (!exhausted && idle != 0 && (__popcll(idle) >= (uint32_t)tune.refillMin || idle == ~0ull)) {
-            while (idle != 0 && !exhausted) {
-                if (qBase >= qEnd) {
-                    uint32_t got = 0;
-                    if (lane == 0) got = atomicAdd(io.head, (uint32_t)tune.chunk);
-                    got = __shfl(got, 0);
-                    if (got >= nIn) {
-                        exhausted = true;
-                        break;
+        if ((top != 0 || !exhausted) && idle != 0 &&
+            (__popcll(idle) >= (uint32_t)tune.refillMin || idle == ~0ull)) {
+            while (idle != 0) {
+                const bool fromStack = top != 0;
+                uint32_t avail;
+                if (fromStack) {
+                    avail = top;
+                } else {
+                    if (exhausted) break;
+                    if (qBase >= qEnd) {
+                        uint32_t got = 0;
+                        if (lane == 0) got = atomicAdd(io.head, (uint32_t)tune.chunk);
+                        got = __shfl(got, 0);
+                        if (got >= nIn) {
+                            exhausted = true;
+                            break;
+                        }
+                        qBase = got;
+                        qEnd = min(got + (uint32_t)tune.chunk, nIn);
                     }
-                    qBase = got;
-                    qEnd = min(got + (uint32_t)tune.chunk, nIn);
+                    avail = qEnd - qBase;
                 }
-                uint32_t avail = qEnd - qBase;
-                uint32_t nIdle = __popcll(idle);
-                uint32_t rank = __popcll(idle & lt);
-                if (!active && rank < avail) {
+                const Particles src = fromStack ? stack : io.in;
+                const uint32_t nIdle = __popcll(idle);
+                const uint32_t take = min(avail, nIdle);
+                const uint32_t rank = __popcll(idle & lt);
+                if (!active && rank < take) {
                     // MC_Load_Particle (MC_Load_Particle.cc:11-29) +
                     // MC_Particle(const MC_Base_Particle&) (MC_Base_Particle.hh:281-320)
-                    uint32_t idx = qBase + rank;
-                    L.x = io.in.x()[idx];
-                    L.y = io.in.y()[idx];
-                    L.z = io.in.z()[idx];
-                    L.vx = io.in.vx()[idx];
-                    L.vy = io.in.vy()[idx];
-                    L.vz = io.in.vz()[idx];
-                    L.E = io.in.E()[idx];
-                    L.w = io.in.w()[idx];
-                    L.tc = io.in.tc()[idx];
-                    L.nmfp = io.in.nmfp()[idx];
-                    L.seed = io.in.seed()[idx];
-                    int cell = io.in.cell()[idx];
+                    const uint32_t idx = fromStack ? top - 1 - rank : qBase + rank;
+                    L.x = src.x()[idx];
+                    L.y = src.y()[idx];
+                    L.z = src.z()[idx];
+                    L.vx = src.vx()[idx];
+                    L.vy = src.vy()[idx];
+                    L.vz = src.vz()[idx];
+                    L.E = src.E()[idx];
+                    L.w = src.w()[idx];
+                    L.tc = src.tc()[idx];
+                    L.nmfp = src.nmfp()[idx];
+                    L.seed = src.seed()[idx];
+                    int cell = src.cell()[idx];
                     L.ci = cell % P.nx;
                     L.cj = (cell / P.nx) % P.ny;
                     L.ck = cell / (P.nx * P.ny);
@@ -677,23 +784,15 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
                     histSeg = 0;
                     active = true;
                 }
-                qBase += min(avail, nIdle);
+                if (fromStack) top -= take;
+                else qBase += take;
                 idle = __ballot(!active);
             }
+            if (lane == 0) *sTopW = top;
         }
-        if (__ballot(active) == 0) {
-            // queue drained.  A single queue ends here (its tallies are
-            // flushed below the loop); a batch flushes this problem's
-            // tallies and moves on to the next queue.
-            if (!Q::kMulti) break;
-            flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
-            if (!acc.next()) break;
-            io = acc.queue();
-            nIn = io.nIn;
-            qBase = qEnd = 0;
-            exhausted = false;
-            continue;
-        }
+        // room for this iteration's pushes (uniform: decided before any lane diverges)
+        const bool stackOk = top + kStackMargin <= scap;
+        if (__ballot(active) == 0) break;  // drained
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
         if (active) {
@@ -801,8 +900,12 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
             if (nOut == 0) {
                 active = false;
             } else {
+                // outputs of a multi-particle collision go to the wave's
+                // stack when it has room, else to the pass's extra queue
+                const Particles dst = stackOk ? stack : io.extra;
+                const uint32_t dcap = stackOk ? scap : io.extraCap;
                 uint32_t slot = 0;
-                if (nOut > 1) slot = wave_append(io.nExtra, (uint32_t)nOut);
+                if (nOut > 1) slot = stackOk ? wave_push(sTopW, (uint32_t)nOut) : wave_append(io.nExtra, (uint32_t)nOut);
                 // secondaries k = 1..nOut-1 are copies of the pre-collision
                 // parent with spawned seeds (CollisionEvent.cc:125-133); the
                 // parent itself (k = nOut, outgoing pair 0) is updated last
@@ -835,14 +938,14 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
                         L = S;
                     } else {
                         uint32_t o = slot + (uint32_t)(k - 1);
-                        if (o < io.extraCap) store_particle(io.extra, o, S, cell);
+                        if (o < dcap) store_particle(dst, o, S, cell);
                         else atomicOr(io.error, 2u);
                     }
                 }
                 L.speed = vlength(L.vx, L.vy, L.vz);
                 if (nOut > 1) {
                     uint32_t o = slot + (uint32_t)(nOut - 1);
-                    if (o < io.extraCap) store_particle(io.extra, o, L, cell);
+                    if (o < dcap) store_particle(dst, o, L, cell);
                     else atomicOr(io.error, 2u);
                     active = false;
                 } else {
@@ -907,11 +1010,32 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune) {
                  __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
     }
     flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
-
 }
 
-template <bool kLds>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_track(DevProblem P, TrackTuning tune, Particles in,
+// Tracks every particle of the wave's queue(s) to the end of its history
+// in this pass (CycleTrackingGuts, CycleTracking.cc:15-119): one queue
+// (k_track) or, round robin, every problem of a batch with queued particles
+// (k_batch_track).  Secondaries go to the wave's stack and are tracked
+// before the wave takes new queue entries.
+template <class Q>
+__device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune, const WaveStacks& ws) {
+    __shared__ uint32_t sTop[4];
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const uint32_t scap = (gw < ws.nWaves) ? ws.cap : 0u;
+    const StackRef sr = {{ws.base + (size_t)gw * 12 * scap * 8, (uint64_t)scap * 8}, scap, &sTop[wib]};
+    if (__lane_id() == 0) sTop[wib] = 0;
+    do {
+        acc.stage();
+        track_queue(acc.prob(), acc.tables(), acc.queue(), tune, sr);
+    } while (Q::kMulti && acc.next());
+}
+
+// kMinWaves: waves per SIMD the register allocation must allow (2: up to
+// 256 VGPRs, 3: 168, 4: 128); a tuning knob, since a tighter budget trades
+// occupancy against spills.
+template <bool kLds, int kMinWaves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves))) k_track(DevProblem P, TrackTuning tune, WaveStacks ws, Particles in,
                                                const uint32_t* __restrict__ nInPtr, Particles census,
                                                uint32_t censusCap, Particles extra, uint32_t* nExtraOut,
                                                uint32_t extraCap, PassCounters* pc, DevCounters* dc) {
@@ -927,7 +1051,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     QueueIO io = {in, *nInPtr, &pc->head, census, &pc->nCensus, censusCap, extra, nExtraOut, extraCap,
                   &pc->error, dc};
     SingleQueue acc = {P, T, io};
-    run_queue(acc, tune);
+    run_queue(acc, tune, ws);
 }
 
 // MC_SourceNow particle generation (MC_SourceNow.cc:68-151) with
@@ -1342,12 +1466,16 @@ __global__ void k_batch_pass_prep(BatchSlot* slots, int n, int passKind) {
 // problem and moves round-robin through the problems, draining one
 // problem's queue at a time (so its descriptor and tables stay
 // wave-uniform); it exits after a full round finds no queued particle.
-__global__ void __launch_bounds__(256)
-k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune) {
+template <int kMinWaves>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
+k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune, WaveStacks ws) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    BatchQueue acc = {slots, n, passKind, (int)(wave % (uint32_t)n)};
+    __shared__ MatInfo sMat[4][kMaxMaterials];
+    __shared__ int32_t sReact[4][kMaxMaterials * kMaxReactions];
+    BatchQueue acc = {slots, n, passKind, (int)(wave % (uint32_t)n), sMat[threadIdx.x >> 6],
+                      sReact[threadIdx.x >> 6]};
     if (!slots[acc.p].st->active && !acc.next()) return;
-    run_queue(acc, tune);
+    run_queue(acc, tune, ws);
 }
 
 // Flux sum per problem in a fixed order (Tallies.cc:146-171), flux reset
@@ -1448,28 +1576,43 @@ void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, u
                        pc, dc);
 }
 
-constexpr size_t kLdsLimit = 64 * 1024;
+constexpr size_t kLdsLimit = 62 * 1024;  // 64 KiB per workgroup less the static tables
 
-void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune,
+bool tablesFitLds(const DevProblem& P) { return tableLdsBytes(P) <= kLdsLimit; }
+
+long long trackPassBlocks(const LaunchCtx& ctx, const TrackTuning& tune, uint32_t nIn) {
+    const int block = tune.blockSize;
+    const int wavesPerBlock = block / 64;
+    long long persistentBlocks = (long long)ctx.numCUs * tune.wavesPerCU / wavesPerBlock;
+    long long needed = ((long long)nIn + block - 1) / block;
+    long long blocks = needed < persistentBlocks ? needed : persistentBlocks;
+    return blocks < 1 ? 1 : blocks;
+}
+
+long long batchTrackBlocks(const LaunchCtx& ctx, const TrackTuning& tune) {
+    long long blocks = (long long)ctx.numCUs * tune.wavesPerCU / (tune.blockSize / 64);
+    return blocks < 1 ? 1 : blocks;
+}
+
+void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune, const WaveStacks& ws,
                      Particles in, const uint32_t* nIn, uint32_t nInHost, Particles census,
                      uint32_t censusCap, Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
                      PassCounters* pc, DevCounters* dc) {
     hipStream_t s = (hipStream_t)ctx.stream;
     ensureCodes(s);
     const int block = tune.blockSize;
-    const int wavesPerBlock = block / 64;
-    long long persistentBlocks = (long long)ctx.numCUs * tune.wavesPerCU / wavesPerBlock;
-    long long needed = ((long long)nInHost + block - 1) / block;
-    long long blocks = needed < persistentBlocks ? needed : persistentBlocks;
-    if (blocks < 1) blocks = 1;
-    size_t lds = tableLdsBytes(P);
-    if (lds <= kLdsLimit) {
-        hipLaunchKernelGGL(k_track<true>, dim3((unsigned)blocks), dim3(block), lds, s, P, tune, in, nIn,
-                           census, censusCap, extra, nExtraOut, extraCap, pc, dc);
-    } else {
-        hipLaunchKernelGGL(k_track<false>, dim3((unsigned)blocks), dim3(block), 0, s, P, tune, in, nIn,
-                           census, censusCap, extra, nExtraOut, extraCap, pc, dc);
+    const long long blocks = trackPassBlocks(ctx, tune, nInHost);
+    const size_t tableBytes = tableLdsBytes(P);
+    const bool inLds = tableBytes <= kLdsLimit;
+    const size_t lds = inLds ? tableBytes : 0;
+    decltype(&k_track<true, 3>) kern;
+    switch (tune.wavesPerSimd) {
+        case 2: kern = inLds ? k_track<true, 2> : k_track<false, 2>; break;
+        case 4: kern = inLds ? k_track<true, 4> : k_track<false, 4>; break;
+        default: kern = inLds ? k_track<true, 3> : k_track<false, 3>; break;
     }
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), lds, s, P, tune, ws, in, nIn, census, censusCap,
+                       extra, nExtraOut, extraCap, pc, dc);
 }
 
 void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partials, int nPartials,
@@ -1508,12 +1651,16 @@ void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int pass
     hipLaunchKernelGGL(k_batch_pass_prep, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, slots, n, passKind);
 }
 
-void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune) {
-    const int wavesPerBlock = tune.blockSize / 64;
-    long long blocks = (long long)ctx.numCUs * tune.wavesPerCU / wavesPerBlock;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_batch_track, dim3((unsigned)blocks), dim3(tune.blockSize), 0, (hipStream_t)ctx.stream,
-                       slots, n, passKind, tune);
+void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
+                      const WaveStacks& ws) {
+    decltype(&k_batch_track<3>) kern;
+    switch (tune.wavesPerSimd) {
+        case 2: kern = k_batch_track<2>; break;
+        case 4: kern = k_batch_track<4>; break;
+        default: kern = k_batch_track<3>; break;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)batchTrackBlocks(ctx, tune)), dim3(tune.blockSize), 0,
+                       (hipStream_t)ctx.stream, slots, n, passKind, tune, ws);
 }
 
 void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,

@@ -188,6 +188,7 @@ void destroyProblem(qsg_problem_s* p) {
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void* a : p->allocs) (void)hipFree(a);
+    p->stacks.release();
     if (p->hpc) (void)hipHostFree(p->hpc);
     if (p->hdc) (void)hipHostFree(p->hdc);
     if (p->hFluxSum) (void)hipHostFree(p->hFluxSum);
@@ -254,7 +255,10 @@ void cycleTracking(qsg_problem_s* p) {
             p->passEv.push_back(e);
         }
         hipCheck(hipEventRecord(p->passEv[2 * k], p->stream), "hipEventRecord");
-        qs::launchTrackPass(c, p->dp, p->tune, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap,
+        const long long blocks = qs::trackPassBlocks(c, p->tune, nInHost);
+        const qs::WaveStacks ws =
+            p->stacks.ensure((uint32_t)(blocks * (p->tune.blockSize / 64)), (uint32_t)p->tune.stackCap, p->stream);
+        qs::launchTrackPass(c, p->dp, p->tune, ws, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap,
                             p->pc, p->dc);
         hipCheck(hipGetLastError(), "track pass launch");
         hipCheck(hipEventRecord(p->passEv[2 * k + 1], p->stream), "hipEventRecord");
@@ -342,10 +346,54 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
 }
 
 qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
-    return qsgi::createFromTables(params, qs::buildProblem(params), device);
+    return qsgi::createFromTables(params, qs::buildProblem(params), device, true);
 }
 
 }  // namespace
+
+qs::WaveStacks qsgi::StackPool::ensure(uint32_t nWaves, uint32_t cap, hipStream_t stream) {
+    if (cap == 0 || nWaves == 0) return qs::WaveStacks{nullptr, 0, 0};
+    const size_t need = qs::waveStackBytes(nWaves, cap);
+    if (need > bytes) {
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        release();
+        void* m = nullptr;
+        hipCheck(hipMalloc(&m, need), "hipMalloc (wave stacks)");
+        mem = static_cast<char*>(m);
+        bytes = need;
+    }
+    return qs::WaveStacks{mem, cap, nWaves};
+}
+
+void qsgi::StackPool::release() {
+    if (mem) (void)hipFree(mem);
+    mem = nullptr;
+    bytes = 0;
+}
+
+qs::TrackTuning qsgi::tuningFromC(const qsg_tuning* t) {
+    if (t->block_size < 64 || t->block_size > 256 || t->block_size % 64)
+        throw QsgError(QSG_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 64) throw QsgError(QSG_ERR_ARG, "waves_per_cu out of range");
+    if (t->refill_min < 1 || t->refill_min > 64) throw QsgError(QSG_ERR_ARG, "refill_min out of range");
+    if (t->chunk < 1 || t->chunk > 4096) throw QsgError(QSG_ERR_ARG, "chunk out of range");
+    if (t->stack_entries != 0 && (t->stack_entries < (int)qs::kStackMargin * 2 || t->stack_entries > 65536))
+        throw QsgError(QSG_ERR_ARG, "stack_entries must be 0 or 512..65536");
+    if (t->sort_interval < 0) throw QsgError(QSG_ERR_ARG, "sort_interval must be >= 0");
+    if (t->waves_per_simd < 2 || t->waves_per_simd > 4) throw QsgError(QSG_ERR_ARG, "waves_per_simd must be 2, 3 or 4");
+    return qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk, t->stack_entries,
+                           t->sort_interval, t->waves_per_simd};
+}
+
+void qsgi::tuningToC(const qs::TrackTuning& in, qsg_tuning* t) {
+    t->block_size = in.blockSize;
+    t->waves_per_cu = in.wavesPerCU;
+    t->refill_min = in.refillMin;
+    t->chunk = in.chunk;
+    t->stack_entries = in.stackCap;
+    t->sort_interval = in.sortInterval;
+    t->waves_per_simd = in.wavesPerSimd;
+}
 
 uint32_t qsgi::particleCapacity(const qs::ProblemTables& T) {
     double factor = 2.0;
@@ -366,12 +414,21 @@ uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T) {
     return b + 64 * 1024;
 }
 
-qsg_problem_s* qsgi::createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device) {
+qsg_problem_s* qsgi::createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device,
+                                      bool ownStacks) {
     qsg_problem_s* p = new qsg_problem_s();
     try {
         p->params = params;
         p->tables = std::move(tables);
         build(p, device);
+        // Wave stacks for this problem's own tracking launches, sized for the
+        // largest grid its population can fill (allocated now so that running
+        // the problem allocates nothing unless its launch shape grows).
+        if (ownStacks && p->tune.stackCap > 0) {
+            const uint64_t maxWaves = (uint64_t)p->numCUs * (uint64_t)p->tune.wavesPerCU;
+            const uint64_t fill = ((uint64_t)p->pop.cap + 255) / 256 * 4;  // any block size
+            p->stacks.ensure((uint32_t)(fill < maxWaves ? fill : maxWaves), (uint32_t)p->tune.stackCap, p->stream);
+        }
     } catch (...) {
         destroyProblem(p);
         throw;
@@ -433,29 +490,21 @@ int qsg_get_info(qsg_problem p, qsg_info* info) {
         info->dt = T.dt;
         info->source_weight = T.sourceWeight;
         info->cycle = p->cycle;
-        info->lds_tables = qs::tableLdsBytes(p->dp) <= 64 * 1024 ? 1 : 0;
+        info->lds_tables = qs::tablesFitLds(p->dp) ? 1 : 0;
     })
 }
 
 int qsg_set_tuning(qsg_problem p, const qsg_tuning* t) {
     QSG_GUARD({
         if (!p || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
-        if (t->block_size < 64 || t->block_size > 256 || t->block_size % 64)
-            throw QsgError(QSG_ERR_ARG, "block_size must be 64, 128, 192 or 256");
-        if (t->waves_per_cu < 1 || t->waves_per_cu > 64) throw QsgError(QSG_ERR_ARG, "waves_per_cu out of range");
-        if (t->refill_min < 1 || t->refill_min > 64) throw QsgError(QSG_ERR_ARG, "refill_min out of range");
-        if (t->chunk < 1 || t->chunk > 4096) throw QsgError(QSG_ERR_ARG, "chunk out of range");
-        p->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+        p->tune = qsgi::tuningFromC(t);
     })
 }
 
 int qsg_get_tuning(qsg_problem p, qsg_tuning* t) {
     QSG_GUARD({
         if (!p || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
-        t->block_size = p->tune.blockSize;
-        t->waves_per_cu = p->tune.wavesPerCU;
-        t->refill_min = p->tune.refillMin;
-        t->chunk = p->tune.chunk;
+        qsgi::tuningToC(p->tune, t);
     })
 }
 

@@ -29,7 +29,9 @@ struct qsg_batch_s {
     double* hFluxSums = nullptr;
     std::vector<uint32_t> active;
     std::vector<int> expectedCycle;  // each problem's cycle as the batch left it
-    qs::TrackTuning tune = {256, 16, 16, 64};
+    qs::TrackTuning tune = qs::kDefaultTuning;
+    qsgi::StackPool ownStacks;
+    qsgi::StackPool* stacks = &ownStacks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> passEv;
     qsg_stats stats = {};
@@ -63,13 +65,15 @@ void destroyBatch(qsg_batch_s* b) {
     for (hipEvent_t e : {b->ev0, b->ev1})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->passEv) (void)hipEventDestroy(e);
+    b->ownStacks.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
-qsg_batch_s* createBatch(qsg_problem* problems, int n) {
+qsg_batch_s* createBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
     if (n <= 0 || !problems) throw QsgError(QSG_ERR_ARG, "a batch needs at least one problem");
     qsg_batch_s* b = new qsg_batch_s();
+    if (pool) b->stacks = pool;
     try {
         b->device = problems[0]->device;
         for (int i = 0; i < n; i++) {
@@ -80,6 +84,12 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n) {
             if (p->cur != (p->cycle & 1))
                 throw QsgError(QSG_ERR_STATE, "problem buffer parity does not match its cycle");
             b->probs.push_back(p);
+        }
+        // members track through the batch's stacks; a problem run alone
+        // again reallocates its own on its next tracking pass
+        for (qsg_problem_s* p : b->probs) {
+            hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+            p->stacks.release();
         }
         hipCheck(hipSetDevice(b->device), "hipSetDevice");
         hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -180,7 +190,10 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
         }
         qs::launchBatchPassPrep(c, b->dSlots, n, kind);
         hipCheck(hipEventRecord(b->passEv[2 * k], b->stream), "hipEventRecord");
-        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune);
+        const long long blocks = qs::batchTrackBlocks(c, b->tune);
+        const qs::WaveStacks ws = b->stacks->ensure((uint32_t)(blocks * (b->tune.blockSize / 64)),
+                                                    (uint32_t)b->tune.stackCap, b->stream);
+        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune, ws);
         hipCheck(hipGetLastError(), "batch track launch");
         hipCheck(hipEventRecord(b->passEv[2 * k + 1], b->stream), "hipEventRecord");
         readStates(b);
@@ -260,7 +273,9 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
 
 }  // namespace
 
-qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n) { return createBatch(problems, n); }
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
+    return createBatch(problems, n, pool);
+}
 void qsgiDestroyBatch(qsg_batch_s* b) { destroyBatch(b); }
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) { batchStep(b, balances, fluxSums); }
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t) { b->tune = t; }
@@ -272,7 +287,7 @@ int qsg_batch_create(qsg_problem* problems, int n, qsg_batch* out) {
     QSG_GUARD({
         if (!out) throw QsgError(QSG_ERR_ARG, "out is NULL");
         *out = nullptr;
-        *out = createBatch(problems, n);
+        *out = createBatch(problems, n, nullptr);
     })
 }
 
@@ -301,12 +316,7 @@ int qsg_batch_set_active(qsg_batch b, int index, int active) {
 int qsg_batch_set_tuning(qsg_batch b, const qsg_tuning* t) {
     QSG_GUARD({
         if (!b || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
-        if (t->block_size < 64 || t->block_size > 256 || t->block_size % 64)
-            throw QsgError(QSG_ERR_ARG, "block_size must be 64, 128, 192 or 256");
-        if (t->waves_per_cu < 1 || t->waves_per_cu > 64) throw QsgError(QSG_ERR_ARG, "waves_per_cu out of range");
-        if (t->refill_min < 1 || t->refill_min > 64) throw QsgError(QSG_ERR_ARG, "refill_min out of range");
-        if (t->chunk < 1 || t->chunk > 4096) throw QsgError(QSG_ERR_ARG, "chunk out of range");
-        b->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+        b->tune = qsgi::tuningFromC(t);
     })
 }
 

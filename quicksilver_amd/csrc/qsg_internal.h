@@ -32,6 +32,17 @@ struct PBuf {
 // Records an error message for qsg_last_error() and returns its code.
 int fail(const std::exception& e);
 
+// Device memory for the per-wave secondary stacks of tracking launches,
+// grown on demand; the owner's stream is synchronised before memory that a
+// launch may still use is freed.
+struct StackPool {
+    char* mem = nullptr;
+    size_t bytes = 0;
+    // Stacks for a launch of nWaves waves with cap entries per wave.
+    qs::WaveStacks ensure(uint32_t nWaves, uint32_t cap, hipStream_t stream);
+    void release();
+};
+
 }  // namespace qsgi
 
 struct qsg_problem_s {
@@ -63,7 +74,8 @@ struct qsg_problem_s {
     double* fluxSumDev = nullptr;
     double* hFluxSum = nullptr;
 
-    qs::TrackTuning tune = {256, 16, 16, 64};
+    qs::TrackTuning tune = qs::kDefaultTuning;
+    qsgi::StackPool stacks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evI0 = nullptr, evI1 = nullptr, evF0 = nullptr, evF1 = nullptr;
     std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
     double lastTrackingSeconds = 0.0;
@@ -91,16 +103,23 @@ struct qsg_problem_s {
 };
 
 namespace qsgi {
+// qsg_tuning <-> launch shape, with range checks (throws QSG_ERR_ARG).
+qs::TrackTuning tuningFromC(const qsg_tuning* t);
+void tuningToC(const qs::TrackTuning& in, qsg_tuning* out);
 // Particle-buffer capacity and total device bytes a problem will allocate.
 uint32_t particleCapacity(const qs::ProblemTables& T);
 uint64_t deviceBytesEstimate(const qs::ProblemTables& T);
 // Uploads already-built tables and allocates the device state.
-qsg_problem_s* createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device);
+// ownStacks: allocate wave stacks for the problem's own tracking launches
+// (problems that only run inside a batch use the batch's stacks).
+qsg_problem_s* createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device,
+                                bool ownStacks);
 void destroy(qsg_problem_s* p);
 }  // namespace qsgi
 
 // Batch internals used by the serving loop.
-qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n);
+// pool: stacks shared with the caller (NULL: the batch owns its stacks)
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, qsgi::StackPool* pool);
 void qsgiDestroyBatch(qsg_batch_s* b);
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums);
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t);

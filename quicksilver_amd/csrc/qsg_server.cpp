@@ -45,8 +45,9 @@ struct qsg_server_s {
     std::map<uint64_t, std::unique_ptr<Request>> all;
     qsg_batch_s* batch = nullptr;
     bool dirty = true;  // membership changed since the batch was built
-    qs::TrackTuning tune = {256, 16, 16, 64};
+    qs::TrackTuning tune = qs::kDefaultTuning;
     bool haveTune = false;
+    qsgi::StackPool stacks;  // shared by the successive batches
     qsg_server_stats stats = {};
 };
 
@@ -83,7 +84,7 @@ void admit(qsg_server_s* s) {
         if (s->maxActive > 0 && (int)s->active.size() >= s->maxActive) break;
         s->queue.pop_front();
         try {
-            r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device);
+            r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device, false);
             r->tables = qs::ProblemTables();
             s->used += r->bytes;
             r->state = 1;
@@ -105,7 +106,7 @@ void serverStep(qsg_server_s* s) {
         dropBatch(s);
         std::vector<qsg_problem> hs;
         for (Request* r : s->active) hs.push_back(r->prob);
-        s->batch = qsgiCreateBatch(hs.data(), (int)hs.size());
+        s->batch = qsgiCreateBatch(hs.data(), (int)hs.size(), &s->stacks);
         if (s->haveTune) qsgiBatchSetTuning(s->batch, s->tune);
         s->dirty = false;
     }
@@ -170,6 +171,8 @@ int qsg_server_destroy(qsg_server s) {
         if (!s) return QSG_OK;
         dropBatch(s);
         for (Request* r : s->active) retire(s, r, 3);
+        (void)hipSetDevice(s->device);
+        s->stacks.release();
         delete s;
     })
 }
@@ -247,7 +250,7 @@ int qsg_server_release(qsg_server s, uint64_t id) {
 int qsg_server_set_tuning(qsg_server s, const qsg_tuning* t) {
     QSG_GUARD({
         if (!s || !t) throw QsgError(QSG_ERR_ARG, "NULL argument");
-        s->tune = qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk};
+        s->tune = qsgi::tuningFromC(t);
         s->haveTune = true;
         if (s->batch) qsgiBatchSetTuning(s->batch, s->tune);
     })

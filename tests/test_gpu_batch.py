@@ -21,12 +21,17 @@ def _check(name, g, got):
 
 
 @pytest.mark.gpu
-def test_mixed_batch_matches_reference_per_problem():
-    print("[batch] mixed fixture decks", flush=True)
+@pytest.mark.parametrize("shape", [{}, {"waves_per_simd": 2}, {"waves_per_simd": 4, "stack_entries": 0},
+                                   {"block_size": 64, "waves_per_cu": 3, "chunk": 8}],
+                         ids=["default", "registers_2_waves", "registers_4_waves_passes_only", "few_waves"])
+def test_mixed_batch_matches_reference_per_problem(shape):
+    print(f"[batch] mixed fixture decks, tuning {shape}", flush=True)
     golds = [load_golden(n) for n in MIX]
     probs = [q.Problem.from_file(g["deck_path"]) for g in golds]
     try:
         with q.Batch(probs) as b:
+            if shape:
+                b.set_tuning(**shape)
             out = b.run([len(g["cycles"]) for g in golds])
         for name, g, got in zip(MIX, golds, out):
             _check(name, g, got)

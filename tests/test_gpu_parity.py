@@ -39,6 +39,40 @@ def test_balance_and_flux_match_reference(name):
             assert abs(flux - rf) <= FLUX_RTOL * abs(rf), f"cycle {c} flux {flux!r} vs {rf!r}"
 
 
+def _check_against(g, out, label):
+    assert len(out) == len(g["cycles"])
+    for c, (ref, (bal, flux, _)) in enumerate(zip(g["cycles"], out)):
+        for f in q.BALANCE_FIELDS:
+            assert bal[f] == ref[f], f"{label} cycle {c} {f}: device {bal[f]} reference {ref[f]}"
+        rf = ref["flux_sum"]
+        assert abs(flux - rf) <= FLUX_RTOL * abs(rf), f"{label} cycle {c} flux {flux!r} vs {rf!r}"
+
+
+# Launch shapes that change which wave tracks which particle, when
+# secondaries are tracked (same launch from a wave stack, or a later pass)
+# and how much of a stack is usable; none may change a tally.
+SHAPES = {
+    "passes_only": {"stack_entries": 0},
+    "small_stack": {"stack_entries": 512, "block_size": 64, "waves_per_cu": 2},
+    "big_stack_fine_refill": {"stack_entries": 4096, "refill_min": 1, "chunk": 16},
+    "coarse_chunks": {"block_size": 128, "waves_per_cu": 32, "refill_min": 32, "chunk": 1024},
+    "registers_2_waves": {"waves_per_simd": 2},
+    "registers_4_waves": {"waves_per_simd": 4, "waves_per_cu": 16},
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", sorted(SHAPES))
+@pytest.mark.parametrize("name", ["small_cube_short", "nested_small", "spheres"])
+def test_launch_shape_does_not_change_tallies(name, shape):
+    print(f"[parity] deck={name} shape={shape}", flush=True)
+    g = load_golden(name)
+    with q.Problem.from_file(g["deck_path"]) as p:
+        p.set_tuning(**SHAPES[shape])
+        out = p.run()
+    _check_against(g, out, shape)
+
+
 @pytest.mark.gpu
 def test_tiny_cube_flux_per_bin_and_census_particles():
     """Per-(cell, group) flux of the last cycle and every census particle

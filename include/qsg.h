@@ -83,6 +83,9 @@ typedef struct qsg_tuning {
                               (0: never)                                     */
     int32_t waves_per_simd; /* register budget of the tracking kernels: 2, 3
                                or 4 resident waves per SIMD                  */
+    int32_t graphs;        /* batched form: 1 replays each step's launches
+                              from captured HIP graphs, 0 launches them one
+                              by one                                         */
 } qsg_tuning;
 
 const char* qsg_last_error(void);
@@ -113,7 +116,10 @@ int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sum
 /* Device timings accumulated since creation (or the last qsg_reset_stats),
  * measured with HIP events on the problem's stream. */
 typedef struct qsg_stats {
-    double track_kernel_seconds; /* sum over tracking-pass kernel launches   */
+    double track_kernel_seconds; /* sum over tracking-pass kernel launches (a
+                                    batch replaying graphs: the captured first
+                                    half of each step, which ends with the
+                                    first pass)                              */
     double tracking_seconds;     /* cycle tracking phases, first to last pass */
     double init_seconds;         /* source + population control              */
     double finalize_seconds;     /* flux reduction + counter readback        */

@@ -82,7 +82,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
                 ("refill_min", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("stack_entries", ctypes.c_int32), ("sort_interval", ctypes.c_int32),
-                ("waves_per_simd", ctypes.c_int32)]
+                ("waves_per_simd", ctypes.c_int32), ("graphs", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -90,7 +90,8 @@ class Tuning(ctypes.Structure):
 
 # Launch shape used when none is set (the library's own default).
 DEFAULT_TUNING = {"block_size": 256, "waves_per_cu": 16, "refill_min": 16, "chunk": 64,
-                  "stack_entries": 512, "sort_interval": 0, "waves_per_simd": 3}
+                  "stack_entries": 512, "sort_interval": 0, "waves_per_simd": 3,
+                  "graphs": 1}
 
 
 def _tuning(kw: dict) -> Tuning:

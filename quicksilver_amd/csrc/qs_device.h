@@ -109,6 +109,7 @@ struct TrackTuning {
     int stackCap;      // per-wave secondary stack entries (0: extra passes only)
     int sortInterval;  // sort the population by cell every n cycles (0: never)
     int wavesPerSimd;  // register budget of the tracking kernels: 2, 3 or 4 waves per SIMD
+    int graphs;        // batched form: replay each step's launches from captured HIP graphs
 };
 
 // Per-wave secondary stacks of a tracking launch.  Wave w (global wave
@@ -128,7 +129,7 @@ constexpr uint32_t kStackCap = 512;
 // one collision per lane, at most 4 outputs each
 constexpr uint32_t kStackMargin = 256;
 inline size_t waveStackBytes(uint32_t nWaves, uint32_t cap) { return (size_t)nWaves * 12 * cap * 8; }
-constexpr TrackTuning kDefaultTuning = {256, 16, 16, 64, (int)kStackCap, 0, 3};
+constexpr TrackTuning kDefaultTuning = {256, 16, 16, 64, (int)kStackCap, 0, 3, 1};
 
 // ---- batched form: many independent problems per launch ----
 // Per-problem mutable state of a batch (device memory).
@@ -184,6 +185,9 @@ void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, u
 // (pc->nCensus), secondaries to `extra` (*nExtraOut).
 // Blocks a tracking pass over nIn queued particles launches (persistent
 // grid of tune.wavesPerCU waves per CU, fewer for small queues).
+// One-time device set-up (constant tables) for launches on ctx's device;
+// call before the first launch and outside any stream capture.
+int prepareDevice(const LaunchCtx& ctx);  // returns a hipError_t
 bool tablesFitLds(const DevProblem& P);
 long long trackPassBlocks(const LaunchCtx& ctx, const TrackTuning& tune, uint32_t nIn);
 long long batchTrackBlocks(const LaunchCtx& ctx, const TrackTuning& tune);

@@ -1517,7 +1517,8 @@ __global__ void k_batch_flux_final(BatchSlot* slots, int n, int partsPerProblem,
     }
 }
 
-bool g_codesReady = false;
+constexpr int kMaxDevices = 64;
+bool g_codesReady[kMaxDevices] = {};
 
 void facetNodeCodes(uint32_t* code) {
     static const int corner[14][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {1, 1, 0, 0},
@@ -1542,16 +1543,25 @@ void facetNodeCodes(uint32_t* code) {
     }
 }
 
-void ensureCodes(hipStream_t s) {
-    if (g_codesReady) return;
+// Uploads the facet node-code table to the current device once (per
+// device: each holds its own copy of constant memory).
+hipError_t ensureCodes(hipStream_t s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (g_codesReady[dev]) return hipSuccess;
     uint32_t code[24];
     facetNodeCodes(code);
-    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_facetCode), code, sizeof(code), 0, hipMemcpyHostToDevice, s);
-    (void)hipStreamSynchronize(s);
-    g_codesReady = true;
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_facetCode), code, sizeof(code), 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) g_codesReady[dev] = true;
+    return e;
 }
 
 }  // namespace
+
+int prepareDevice(const LaunchCtx& ctx) { return (int)ensureCodes((hipStream_t)ctx.stream); }
 
 size_t tableLdsBytes(const DevProblem& P) {
     return sizeof(double) * ((size_t)(P.G + 1) + (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);
@@ -1560,7 +1570,6 @@ size_t tableLdsBytes(const DevProblem& P) {
 void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, uint32_t outBase,
                   uint64_t nSource, uint32_t cycle) {
     hipStream_t s = (hipStream_t)ctx.stream;
-    ensureCodes(s);
     if (nSource == 0) return;
     unsigned blocks = (unsigned)((nSource + 255) / 256);
     hipLaunchKernelGGL(k_source, dim3(blocks), dim3(256), 0, s, P, out, outBase, nSource, cycle);
@@ -1599,7 +1608,6 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
                      uint32_t censusCap, Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
                      PassCounters* pc, DevCounters* dc) {
     hipStream_t s = (hipStream_t)ctx.stream;
-    ensureCodes(s);
     const int block = tune.blockSize;
     const long long blocks = trackPassBlocks(ctx, tune, nInHost);
     const size_t tableBytes = tableLdsBytes(P);
@@ -1637,7 +1645,6 @@ void launchBatchPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, uint64_t* sr
 
 void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* srcPrefix) {
     hipStream_t s = (hipStream_t)ctx.stream;
-    ensureCodes(s);
     hipLaunchKernelGGL(k_batch_source, dim3(ctx.numCUs * 8), dim3(256), 0, s, slots, n, srcPrefix);
 }
 

@@ -107,6 +107,7 @@ void build(qsg_problem_s* p, int device) {
              "hipDeviceGetAttribute");
     for (hipEvent_t* e : {&p->ev0, &p->ev1, &p->evI0, &p->evI1, &p->evF0, &p->evF1})
         hipCheck(hipEventCreate(e), "hipEventCreate");
+    hipCheck((hipError_t)qs::prepareDevice(p->ctx()), "device set-up");
 
     const qs::ProblemTables& T = p->tables;
     if ((int)T.materials.size() > qs::kMaxMaterials)
@@ -381,8 +382,9 @@ qs::TrackTuning qsgi::tuningFromC(const qsg_tuning* t) {
         throw QsgError(QSG_ERR_ARG, "stack_entries must be 0 or 512..65536");
     if (t->sort_interval < 0) throw QsgError(QSG_ERR_ARG, "sort_interval must be >= 0");
     if (t->waves_per_simd < 2 || t->waves_per_simd > 4) throw QsgError(QSG_ERR_ARG, "waves_per_simd must be 2, 3 or 4");
+    if (t->graphs != 0 && t->graphs != 1) throw QsgError(QSG_ERR_ARG, "graphs must be 0 or 1");
     return qs::TrackTuning{t->block_size, t->waves_per_cu, t->refill_min, t->chunk, t->stack_entries,
-                           t->sort_interval, t->waves_per_simd};
+                           t->sort_interval, t->waves_per_simd, t->graphs};
 }
 
 void qsgi::tuningToC(const qs::TrackTuning& in, qsg_tuning* t) {
@@ -393,6 +395,7 @@ void qsgi::tuningToC(const qs::TrackTuning& in, qsg_tuning* t) {
     t->stack_entries = in.stackCap;
     t->sort_interval = in.sortInterval;
     t->waves_per_simd = in.wavesPerSimd;
+    t->graphs = in.graphs;
 }
 
 uint32_t qsgi::particleCapacity(const qs::ProblemTables& T) {

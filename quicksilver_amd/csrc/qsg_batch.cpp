@@ -34,6 +34,12 @@ struct qsg_batch_s {
     qsgi::StackPool* stacks = &ownStacks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> passEv;
+    // captured launch sequences of a step (first half up to the first
+    // tracking pass, second half from the flux reduction on) and what they
+    // were captured with
+    hipGraphExec_t gInit = nullptr, gFinal = nullptr;
+    qs::TrackTuning graphTune = {};
+    qs::WaveStacks graphStacks = {};
     qsg_stats stats = {};
 };
 
@@ -65,6 +71,8 @@ void destroyBatch(qsg_batch_s* b) {
     for (hipEvent_t e : {b->ev0, b->ev1})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->passEv) (void)hipEventDestroy(e);
+    for (hipGraphExec_t g : {b->gInit, b->gFinal})
+        if (g) (void)hipGraphExecDestroy(g);
     b->ownStacks.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -97,6 +105,7 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
                  "hipDeviceGetAttribute");
         hipCheck(hipEventCreate(&b->ev0), "hipEventCreate");
         hipCheck(hipEventCreate(&b->ev1), "hipEventCreate");
+        hipCheck((hipError_t)qs::prepareDevice(qs::LaunchCtx{(void*)b->stream, b->numCUs}), "device set-up");
         b->tune = problems[0]->tune;
         b->dSlots = devAlloc<qs::BatchSlot>(n);
         b->dState = devAlloc<qs::BatchState>(n);
@@ -165,8 +174,94 @@ void checkBatchErrors(qsg_batch_s* b) {
     }
 }
 
+hipEvent_t passEvent(qsg_batch_s* b, int i) {
+    while ((int)b->passEv.size() <= i) {
+        hipEvent_t e;
+        hipCheck(hipEventCreate(&e), "hipEventCreate");
+        b->passEv.push_back(e);
+    }
+    return b->passEv[i];
+}
+
+// First half of a step: cycle set-up, source and population control of
+// every problem, the first tracking pass, and the state copy to the host.
+// timed: bracket the tracking pass with events (eager launches only; a
+// captured graph is timed as a whole from outside, since event records
+// inside a graph do not yield elapsed times).
+void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws, bool timed) {
+    const int n = (int)b->probs.size();
+    qs::launchBatchPrep(c, b->dSlots, n, b->dSrcPrefix, b->dPopPrefix);
+    qs::launchBatchSource(c, b->dSlots, n, b->dSrcPrefix);
+    qs::launchBatchPopulation(c, b->dSlots, n, b->dPopPrefix);
+    qs::launchBatchPassPrep(c, b->dSlots, n, 0);
+    if (timed) hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
+    qs::launchBatchTrack(c, b->dSlots, n, 0, b->tune, ws);
+    hipCheck(hipGetLastError(), "batch track launch");
+    if (timed) hipCheck(hipEventRecord(b->passEv[1], b->stream), "hipEventRecord");
+    hipCheck(hipMemcpyAsync(b->hState, b->dState, n * sizeof(qs::BatchState), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
+}
+
+// Second half: flux reduction and cycle advance, and the copies of the
+// flux sums, tallies and states to the host.
+void enqueueFinal(qsg_batch_s* b, const qs::LaunchCtx& c) {
+    const int n = (int)b->probs.size();
+    qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
+    hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(b->hCounters, b->dCounters, n * sizeof(qs::DevCounters), hipMemcpyDeviceToHost,
+                            b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(b->hState, b->dState, n * sizeof(qs::BatchState), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
+}
+
+template <class F>
+hipGraphExec_t capture(qsg_batch_s* b, F enqueue) {
+    hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipGraph_t g = nullptr;
+    try {
+        enqueue();
+    } catch (...) {
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hipCheck(hipStreamEndCapture(b->stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hipCheck(err, "hipGraphInstantiate");
+    return e;
+}
+
+bool sameShape(const qs::TrackTuning& a, const qs::TrackTuning& t) {
+    return a.blockSize == t.blockSize && a.wavesPerCU == t.wavesPerCU && a.refillMin == t.refillMin &&
+           a.chunk == t.chunk && a.stackCap == t.stackCap && a.wavesPerSimd == t.wavesPerSimd;
+}
+
+// The step's two graphs, captured again whenever the launch shape or the
+// stack memory changed since they were captured.
+void ensureGraphs(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws) {
+    if (b->gInit && b->gFinal && sameShape(b->graphTune, b->tune) && b->graphStacks.base == ws.base &&
+        b->graphStacks.cap == ws.cap && b->graphStacks.nWaves == ws.nWaves)
+        return;
+    for (hipGraphExec_t* g : {&b->gInit, &b->gFinal})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+    b->gInit = capture(b, [&] { enqueueInit(b, c, ws, false); });
+    b->gFinal = capture(b, [&] { enqueueFinal(b, c); });
+    b->graphTune = b->tune;
+    b->graphStacks = ws;
+}
+
 // One cycle of every active problem (cycleInit + cycleTracking +
-// cycleFinalize of main.cc:55-66, for all of them at once).
+// cycleFinalize of main.cc:55-66, for all of them at once).  Normally two
+// graph launches and two host synchronisations: the first tracking pass
+// tracks secondaries from the waves' stacks, and further passes (launched
+// one by one) are needed only for secondaries that overflowed a stack.
 void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
     hipCheck(hipSetDevice(b->device), "hipSetDevice");
     const int n = (int)b->probs.size();
@@ -174,50 +269,53 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
         if (b->probs[i]->phase != 0 || b->probs[i]->cycle != b->expectedCycle[i])
             throw QsgError(QSG_ERR_STATE, "a batch problem was advanced outside the batch");
     qs::LaunchCtx c{(void*)b->stream, b->numCUs};
+    const long long blocks = qs::batchTrackBlocks(c, b->tune);
+    const qs::WaveStacks ws = b->stacks->ensure((uint32_t)(blocks * (b->tune.blockSize / 64)),
+                                                (uint32_t)b->tune.stackCap, b->stream);
+    passEvent(b, 1);
+    const bool graphs = b->tune.graphs != 0;
+    if (graphs) ensureGraphs(b, c, ws);
+
     hipCheck(hipEventRecord(b->ev0, b->stream), "hipEventRecord");
-    qs::launchBatchPrep(c, b->dSlots, n, b->dSrcPrefix, b->dPopPrefix);
-    qs::launchBatchSource(c, b->dSlots, n, b->dSrcPrefix);
-    qs::launchBatchPopulation(c, b->dSlots, n, b->dPopPrefix);
-    // tracking passes until no problem has secondaries left
-    int k = 0;
-    double trackSeconds = 0;
+    if (graphs) {
+        // the whole first half is timed as the first pass
+        hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
+        hipCheck(hipGraphLaunch(b->gInit, b->stream), "hipGraphLaunch");
+        hipCheck(hipEventRecord(b->passEv[1], b->stream), "hipEventRecord");
+    } else {
+        enqueueInit(b, c, ws, true);
+    }
+    hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    checkBatchErrors(b);
+    // further passes for secondaries that overflowed the stacks
+    int k = 1;
     for (;;) {
-        const int kind = (k == 0) ? 0 : ((k % 2 == 1) ? 1 : 2);
-        while ((int)b->passEv.size() < 2 * (k + 1)) {
-            hipEvent_t e;
-            hipCheck(hipEventCreate(&e), "hipEventCreate");
-            b->passEv.push_back(e);
-        }
-        qs::launchBatchPassPrep(c, b->dSlots, n, kind);
-        hipCheck(hipEventRecord(b->passEv[2 * k], b->stream), "hipEventRecord");
-        const long long blocks = qs::batchTrackBlocks(c, b->tune);
-        const qs::WaveStacks ws = b->stacks->ensure((uint32_t)(blocks * (b->tune.blockSize / 64)),
-                                                    (uint32_t)b->tune.stackCap, b->stream);
-        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune, ws);
-        hipCheck(hipGetLastError(), "batch track launch");
-        hipCheck(hipEventRecord(b->passEv[2 * k + 1], b->stream), "hipEventRecord");
-        readStates(b);
-        checkBatchErrors(b);
-        const int outIdx = (kind == 1) ? 1 : 0;
+        const int outIdx = (k % 2 == 1) ? 0 : 1;  // where pass k-1 wrote its secondaries
         uint64_t pending = 0;
         for (int i = 0; i < n; i++)
             if (b->hState[i].active) pending += b->hState[i].extra[outIdx];
-        k++;
         if (pending == 0) break;
+        const int kind = (k % 2 == 1) ? 1 : 2;
+        hipEvent_t e0 = passEvent(b, 2 * k), e1 = passEvent(b, 2 * k + 1);
+        qs::launchBatchPassPrep(c, b->dSlots, n, kind);
+        hipCheck(hipEventRecord(e0, b->stream), "hipEventRecord");
+        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune, ws);
+        hipCheck(hipGetLastError(), "batch track launch");
+        hipCheck(hipEventRecord(e1, b->stream), "hipEventRecord");
+        readStates(b);
+        checkBatchErrors(b);
+        k++;
     }
+    double trackSeconds = 0;
     for (int q = 0; q < k; q++) {
         float ms = 0;
         hipCheck(hipEventElapsedTime(&ms, b->passEv[2 * q], b->passEv[2 * q + 1]), "hipEventElapsedTime");
         trackSeconds += ms * 1e-3;
     }
-    qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
-    hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
-             "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(b->hCounters, b->dCounters, n * sizeof(qs::DevCounters), hipMemcpyDeviceToHost,
-                            b->stream),
-             "hipMemcpyAsync");
+    if (graphs) hipCheck(hipGraphLaunch(b->gFinal, b->stream), "hipGraphLaunch");
+    else enqueueFinal(b, c);
     hipCheck(hipEventRecord(b->ev1, b->stream), "hipEventRecord");
-    readStates(b);
+    hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     checkBatchErrors(b);
     float stepMs = 0;
     hipCheck(hipEventElapsedTime(&stepMs, b->ev0, b->ev1), "hipEventElapsedTime");

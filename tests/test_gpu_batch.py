@@ -1,7 +1,9 @@
 """Batched form: several problems of uneven size and step count advance
 together in shared launches; every problem must reproduce the reference
 CPU path's tallies bit-exactly (flux to 1e-10), i.e. the same result alone
-and in the mix."""
+and in the mix.  The default form replays each step from captured HIP
+graphs, so every cycle after the first checks a graph replay (with the
+library's own eager launches and host synchronisations in between)."""
 import pytest
 
 import quicksilver_amd as q
@@ -21,9 +23,11 @@ def _check(name, g, got):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [{}, {"waves_per_simd": 2}, {"waves_per_simd": 4, "stack_entries": 0},
+@pytest.mark.parametrize("shape", [{}, {"graphs": 0}, {"waves_per_simd": 2},
+                                   {"waves_per_simd": 4, "stack_entries": 0},
                                    {"block_size": 64, "waves_per_cu": 3, "chunk": 8}],
-                         ids=["default", "registers_2_waves", "registers_4_waves_passes_only", "few_waves"])
+                         ids=["default_graphs", "eager_launches", "registers_2_waves",
+                              "registers_4_waves_passes_only", "few_waves"])
 def test_mixed_batch_matches_reference_per_problem(shape):
     print(f"[batch] mixed fixture decks, tuning {shape}", flush=True)
     golds = [load_golden(n) for n in MIX]

@@ -42,6 +42,23 @@ QS_REF = os.path.join(ROOT, "oracle", "_ref", "qs_ref")
 METRIC = "segments/sec on 1 MI355X, batched vs one-at-a-time; p50/p99 solve latency"
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_track_nested_boxes_traffic.md")
+
+
+def measured_traffic_per_segment():
+    """HBM bytes per segment of k_track on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of
+    known size); None when the profile is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_segment:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def bytes_per_segment(stats: dict, n_iso: int, n_react: int) -> float:
     S = max(stats["segments"], 1)
     f_fc = stats["facet_crossings"] / S
@@ -139,6 +156,23 @@ def serving(n_requests: int, capacity_problems_per_s: float):
     }
 
 
+def copy_probe(gib: float = 2.0, reps: int = 3):
+    """A plain device-to-device copy of known size, run under the same
+    rocprofv3 counters as the tracking kernel: FETCH_SIZE / WRITE_SIZE read
+    for it (per dispatch) calibrate what those counters report for a known
+    number of bytes read and written."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "dispatches": reps}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,7 +184,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 2 GiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the problem is "
               "sharded by particles or by whole problems across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -178,6 +217,7 @@ def main():
     kernel_s = stats["track_kernel_seconds"]
     achieved = bseg * stats["segments"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
     launches = max(stats["track_launches"], 1)
+    traffic_seg = measured_traffic_per_segment()
 
     line = {
         "metric": METRIC,
@@ -206,9 +246,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_seg * stats["segments"] / launches if traffic_seg else None,
+            "traffic_source": "profiles/k_track_nested_boxes_traffic.md: rocprofv3 FETCH_SIZE x 2.0 + "
+                              "WRITE_SIZE x 1.0 (factors from a 2 GiB copy), bytes per segment x the "
+                              "segments of this run's average launch",
             "kernel": "k_track",
             "bytes_per_segment": bseg,
+            "traffic_bytes_per_segment": traffic_seg,
             "segments_per_launch": stats["segments"] / launches,
             "avg_launch_ms": kernel_s / launches * 1e3,
         },

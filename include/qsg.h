@@ -183,6 +183,28 @@ int qsg_server_release(qsg_server s, uint64_t id);
 int qsg_server_set_tuning(qsg_server s, const qsg_tuning* t);
 int qsg_server_get_stats(qsg_server s, qsg_server_stats* stats);
 
+/* CORAL2 correctness checks (coralBenchmarkCorrectness, main.cc:73 ->
+ * CoralBenchmark.cc:17-228) over the balance tallies accumulated since
+ * creation and, for the fluence test, the per-cell fluence accumulated on the
+ * device when the deck sets coralBenchmark. */
+typedef struct qsg_coral_report {
+    int32_t benchmark;      /* the deck's coralBenchmark (1 or 2; 0: none)     */
+    int32_t ratio_pass;     /* BalanceRatioTest      (CoralBenchmark.cc:46)  */
+    int32_t event_pass;     /* BalanceEventTest      (CoralBenchmark.cc:119) */
+    int32_t missing_pass;   /* MissingParticleTest   (CoralBenchmark.cc:152) */
+    int32_t fluence_pass;   /* FluenceTest           (CoralBenchmark.cc:177) */
+    double ratio_error;     /* largest of the six relative ratio errors      */
+    double ratio_tolerance; /* 0.01 (problem 1) or 0.011 (problem 2)         */
+    double event_ratio;     /* |facet crossings / collisions - 1|            */
+    uint64_t gains, losses; /* start+source+produce+split, absorb+census+
+                               escape+rr+fission, summed over all cycles      */
+    double fluence_max_percent_diff;
+} qsg_coral_report;
+int qsg_coral_check(qsg_problem p, qsg_coral_report* report);
+/* Cumulative per-cell fluence (n_cells values, global cell order); only for
+ * decks that set coralBenchmark. */
+int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells);
+
 /* Device FP64 builtins on host arrays (rounding checks).
  * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a) */
 int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device);

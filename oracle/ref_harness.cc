@@ -24,6 +24,8 @@
 //   QSREF_FLUX_DUMP=path   append, per cycle, the scalar flux of every
 //                          (global cell id, group) as raw float64 in
 //                          global-cell-major order (requires one domain).
+//   QSREF_FLUENCE_DUMP=path after the run, the cumulative per-cell fluence
+//                          (coralBenchmark decks), f64, global cell order.
 //   QSREF_CENSUS_DUMP=path write the census particles of the last cycle as
 //                          text: seed x y z vx vy vz E w tc nmfp cell
 #include <chrono>
@@ -49,6 +51,7 @@
 #include "CycleTracking.hh"
 #include "SendQueue.hh"
 #include "MC_Fast_Timer.hh"
+#include "CoralBenchmark.hh"
 
 #ifdef HAVE_OPENMP
 #include <omp.h>
@@ -146,6 +149,22 @@ void dump_flux(MonteCarlo* mc, FILE* f)
     fwrite(out.data(), sizeof(double), out.size(), f);
 }
 
+// Cumulative per-cell fluence (Tallies::_fluence, accumulated by
+// CycleFinalize when coralBenchmark is set), in global cell order.
+void dump_fluence(MonteCarlo* mc, const char* path)
+{
+    Tallies* t = mc->_tallies;
+    if (t->_fluence._domain.size() != 1) { fprintf(stderr, "fluence dump needs one domain\n"); exit(3); }
+    FluenceDomain* fd = t->_fluence._domain[0];
+    MC_Domain& dom = mc->domain[0];
+    std::vector<double> out((size_t)fd->size(), 0.0);
+    for (int c = 0; c < fd->size(); c++) out[dom.cell_state[c]._id >> 32] = fd->getCell(c);
+    FILE* f = fopen(path, "wb");
+    if (!f) return;
+    fwrite(out.data(), sizeof(double), out.size(), f);
+    fclose(f);
+}
+
 void dump_census(MonteCarlo* mc, const char* path)
 {
     FILE* f = fopen(path, "w");
@@ -216,6 +235,11 @@ int main(int argc, char** argv)
     printf("QSREF_FOM %" PRIu64 " %.6e %.6e %d\n", totalSeg, totalTrack,
            totalTrack > 0 ? (double)totalSeg / totalTrack : 0.0, threads);
     fflush(stdout);
+    // main.cc:73: the CORAL2 correctness tests (no-op unless coralBenchmark)
+    coralBenchmarkCorrectness(mcco, params);
+    fflush(stdout);
+    const char* fluencePath = getenv("QSREF_FLUENCE_DUMP");
+    if (fluencePath && params.simulationParams.coralBenchmark) dump_fluence(mcco, fluencePath);
     delete mcco;
     mpiFinalize();
     return 0;

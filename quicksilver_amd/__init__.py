@@ -33,7 +33,8 @@ EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy"
            "qsg_batch_create", "qsg_batch_destroy", "qsg_batch_step", "qsg_batch_set_active",
            "qsg_batch_set_tuning", "qsg_batch_get_stats", "qsg_server_create", "qsg_server_destroy",
            "qsg_server_submit", "qsg_server_step", "qsg_server_status", "qsg_server_result",
-           "qsg_server_release", "qsg_server_set_tuning", "qsg_server_get_stats"]
+           "qsg_server_release", "qsg_server_set_tuning", "qsg_server_get_stats",
+           "qsg_coral_check", "qsg_get_fluence"]
 
 
 class QsgError(RuntimeError):
@@ -57,6 +58,32 @@ class Info(ctypes.Structure):
                 ("particle_capacity", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("dt", ctypes.c_double), ("source_weight", ctypes.c_double),
                 ("cycle", ctypes.c_int32), ("lds_tables", ctypes.c_int32)]
+
+
+class CoralReport(ctypes.Structure):
+    """qsg_coral_report: the reference's CORAL2 correctness tests."""
+    _fields_ = [("benchmark", ctypes.c_int32), ("ratio_pass", ctypes.c_int32), ("event_pass", ctypes.c_int32),
+                ("missing_pass", ctypes.c_int32), ("fluence_pass", ctypes.c_int32),
+                ("ratio_error", ctypes.c_double), ("ratio_tolerance", ctypes.c_double),
+                ("event_ratio", ctypes.c_double), ("gains", ctypes.c_uint64), ("losses", ctypes.c_uint64),
+                ("fluence_max_percent_diff", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    def verdicts(self) -> List[str]:
+        """The PASS/FAIL lines the reference prints (CoralBenchmark.cc)."""
+        tol = f"{self.ratio_tolerance * 100.0:g}"
+        return [
+            f"PASS:: Absorption / Fission / Scatter Ratios maintained with {tol}% tolerance" if self.ratio_pass
+            else f"FAIL:: Absorption / Fission / Scatter Ratios NOT maintained with {tol}% tolerance",
+            "PASS:: Collision to Facet Crossing Ratio maintained even balanced within 1% tolerance"
+            if self.event_pass else "FAIL:: Collision to Facet Crossing Ratio balanced NOT maintained within 1% tolerance",
+            "PASS:: No Particles Lost During Run" if self.missing_pass
+            else "FAIL:: Particles Were Lost During Run, test for done should have failed",
+            "PASS:: Fluence is homogenous across cells with 6% tolerance" if self.fluence_pass
+            else "FAIL:: Fluence not homogenous across cells within 6% tolerance",
+        ]
 
 
 class Stats(ctypes.Structure):
@@ -309,6 +336,17 @@ class Problem:
         out = np.zeros((inf.n_cells, inf.n_groups), dtype=np.float64)
         _check(lib().qsg_get_scalar_flux(self._h, out.ctypes.data_as(_D), out.size))
         return out
+
+    def fluence(self) -> np.ndarray:
+        """Cumulative per-cell fluence (decks with coralBenchmark set)."""
+        out = np.zeros(self.info().n_cells, dtype=np.float64)
+        _check(lib().qsg_get_fluence(self._h, out.ctypes.data_as(_D), out.size))
+        return out
+
+    def coral_check(self) -> CoralReport:
+        r = CoralReport()
+        _check(lib().qsg_coral_check(self._h, ctypes.byref(r)))
+        return r
 
     # ---- particle state
     def set_particles(self, ps: ParticleSet):

@@ -81,6 +81,7 @@ struct DevProblem {
     double dt, eMin, eMax, sourceWeight, lowWeightCutoff;
     // tallies
     double* flux;  // [nCells][G]
+    double* fluence;  // [nCells] cumulative over cycles (CORAL2 decks only, else null)
 };
 
 // Balance counters accumulated on the device (Tallies.hh:36-100 Balance).
@@ -202,6 +203,9 @@ void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partial
                    double* result);
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
+// fluence[cell] += flux[cell][g] for g = 0..G-1 in order (Fluence::compute,
+// Tallies.cc:100-120); call before the flux is reduced and zeroed.
+void launchFluence(const LaunchCtx& ctx, const DevProblem& P);
 
 // Batched launchers over `n` slots.  prefix arrays hold n+1 entries.
 void launchBatchPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, uint64_t* srcPrefix, uint64_t* popPrefix);

@@ -600,6 +600,7 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.sourceWeight = rfld(c.sourceWeight);
     P.lowWeightCutoff = rfld(c.lowWeightCutoff);
     P.flux = rflp(c.flux);
+    P.fluence = rflp(c.fluence);
     return P;
 }
 
@@ -1340,6 +1341,18 @@ __global__ void k_math(int op, const double* a, const double* b, double* out, si
     out[q] = r;
 }
 
+// Fluence::compute (Tallies.cc:100-120): per cell, the groups' flux added
+// one by one to the cumulative value.
+__global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux, double* __restrict__ fluence,
+                                                 int nCells, int G) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nCells; c += gridDim.x * blockDim.x) {
+        double v = fluence[c];
+        const double* f = flux + (size_t)c * G;
+        for (int g = 0; g < G; g++) v += f[g];
+        fluence[c] = v;
+    }
+}
+
 // ------------------------------------------------------ batched form ----
 // Many independent problems advance one cycle together.  Every problem keeps
 // its own particle buffers, tallies and RNG streams, so its results are the
@@ -1678,6 +1691,14 @@ void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* 
     unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_batch_flux_final, dim3(blocks), dim3(256), 0, s, slots, n, partsPerProblem, partials,
                        fluxSums);
+}
+
+void launchFluence(const LaunchCtx& ctx, const DevProblem& P) {
+    if (!P.fluence) return;
+    unsigned blocks = (unsigned)((P.nCells + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fluence, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, P.flux, P.fluence, P.nCells,
+                       P.G);
 }
 
 void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out,

@@ -56,6 +56,34 @@ int main(int argc, char** argv) {
     }
     printf("Figure Of Merit  %14.2e [Num Segments / Cycle Tracking Time]  (%" PRIu64 " segments, %.6e s)\n",
            totalTrack > 0 ? (double)totalSeg / totalTrack : 0.0, totalSeg, totalTrack);
+    // coralBenchmarkCorrectness (main.cc:73): the CORAL2 checks, printed in
+    // the reference's words
+    qsg_coral_report r;
+    if (qsg_coral_check(p, &r) == QSG_OK) {
+        printf("\nTesting Ratios for Absorbtion, Fission, and Scattering are maintained\n");
+        if (r.ratio_pass)
+            printf("PASS:: Absorption / Fission / Scatter Ratios maintained with %g%% tolerance\n",
+                   r.ratio_tolerance * 100.0);
+        else
+            printf("FAIL:: Absorption / Fission / Scatter Ratios NOT maintained with %g%% tolerance\n",
+                   r.ratio_tolerance * 100.0);
+        printf("\nTesting balance between number of facet crossings and reactions\n");
+        if (r.event_pass)
+            printf("PASS:: Collision to Facet Crossing Ratio maintained even balanced within %g%% tolerance\n", 1.0);
+        else
+            printf(" FAIL:: Collision to Facet Crossing Ratio balanced NOT maintained within %g%% tolerance\n", 1.0);
+        printf("\nTest for lost / unaccounted for particles in this simulation\n");
+        if (r.missing_pass)
+            printf("PASS:: No Particles Lost During Run\n");
+        else
+            printf("FAIL:: Particles Were Lost During Run, test for done should have failed\n");
+        printf("\nTest Fluence for homogeneity across cells\n");
+        if (r.fluence_pass)
+            printf("PASS:: Fluence is homogenous across cells with %g%% tolerance\n", 6.0);
+        else
+            printf("FAIL:: Fluence not homogenous across cells within %g%% tolerance\n"
+                   "\tCurrent Max Percent Diff: %4.1f%%\n", 6.0, r.fluence_max_percent_diff);
+    }
     qsg_destroy(p);
     return 0;
 }

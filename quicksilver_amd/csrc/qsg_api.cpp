@@ -162,6 +162,11 @@ void build(qsg_problem_s* p, int device) {
     d.lowWeightCutoff = T.lowWeightCutoff;
     d.flux = p->dalloc<double>((size_t)T.nCells * T.nGroups);
     hipCheck(hipMemset(d.flux, 0, (size_t)T.nCells * T.nGroups * sizeof(double)), "hipMemset");
+    d.fluence = nullptr;
+    if (p->params.simulationParams.coralBenchmark) {
+        d.fluence = p->dalloc<double>((size_t)T.nCells);
+        hipCheck(hipMemset(d.fluence, 0, (size_t)T.nCells * sizeof(double)), "hipMemset");
+    }
 
     // particle buffers
     uint32_t cap = qsgi::particleCapacity(T);
@@ -304,6 +309,7 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
     qs::LaunchCtx c = p->ctx();
     const qs::ProblemTables& T = p->tables;
     hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
+    qs::launchFluence(c, p->dp);
     qs::launchFluxSum(c, p->dp.flux, (size_t)T.nCells * T.nGroups, p->partials, p->nPartials, p->fluxSumDev);
     hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
     hipCheck(hipMemcpyAsync(p->hFluxSum, p->fluxSumDev, sizeof(double), hipMemcpyDeviceToHost, p->stream),
@@ -324,21 +330,22 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
         p->stats.particle_loads += p->startCount + T.sourcePerCycle + d.split - d.rr + d.produce;
         p->stats.cycles += 1;
     }
-    if (bal) {
-        bal->start = p->startCount;
-        bal->source = T.sourcePerCycle;
-        bal->rr = d.rr;
-        bal->split = d.split;
-        bal->absorb = d.absorb;
-        bal->scatter = d.scatter;
-        bal->fission = d.fission;
-        bal->produce = d.produce;
-        bal->collision = d.collision;
-        bal->escape = d.escape;
-        bal->census = d.census;
-        bal->num_segments = d.numSegments;
-        bal->end = p->nCensus;
-    }
+    qsg_balance b;
+    b.start = p->startCount;
+    b.source = T.sourcePerCycle;
+    b.rr = d.rr;
+    b.split = d.split;
+    b.absorb = d.absorb;
+    b.scatter = d.scatter;
+    b.fission = d.fission;
+    b.produce = d.produce;
+    b.collision = d.collision;
+    b.escape = d.escape;
+    b.census = d.census;
+    b.num_segments = d.numSegments;
+    b.end = p->nCensus;
+    qsgi::addBalance(p->cumulative, b);
+    if (bal) *bal = b;
     if (fluxSum) *fluxSum = *p->hFluxSum;
     p->startCount = p->nCensus;
     p->cur = p->censusBuf;
@@ -617,6 +624,81 @@ int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n) {
         if (n > p->nPop) throw QsgError(QSG_ERR_ARG, "n exceeds the processing count");
         setDevice(p);
         copyParticles(p->pop.v, out, n, hipMemcpyDeviceToHost, p->stream);
+    })
+}
+
+int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells) {
+    QSG_GUARD({
+        if (!p || !fluence) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (!p->dp.fluence) throw QsgError(QSG_ERR_STATE, "fluence is accumulated only for coralBenchmark decks");
+        if (n_cells != (uint64_t)p->tables.nCells) throw QsgError(QSG_ERR_ARG, "n_cells does not match the mesh");
+        setDevice(p);
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        hipCheck(hipMemcpy(fluence, p->dp.fluence, n_cells * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    })
+}
+
+int qsg_coral_check(qsg_problem p, qsg_coral_report* r) {
+    QSG_GUARD({
+        if (!p || !r) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *r = qsg_coral_report{};
+        const int bench = p->params.simulationParams.coralBenchmark;
+        r->benchmark = bench;
+        if (bench != 1 && bench != 2) throw QsgError(QSG_ERR_STATE, "the deck does not set coralBenchmark 1 or 2");
+        const qsg_balance& b = p->cumulative;
+        // BalanceRatioTest (CoralBenchmark.cc:46-117): the expected reaction
+        // ratios of each CORAL2 problem, every pairwise relative error
+        double fissionRatio, scatterRatio, absorbRatio, tol;
+        if (bench == 1) {
+            fissionRatio = 0.05;
+            scatterRatio = 1;
+            absorbRatio = 0.04;
+            tol = 1.0 / 100.0;
+        } else {
+            fissionRatio = 0.075;
+            scatterRatio = 0.830;
+            absorbRatio = 0.094;
+            tol = 1.1 / 100.0;
+        }
+        const uint64_t absorb = b.absorb, fission = b.fission, scatter = b.scatter;
+        const double errs[6] = {std::abs((absorb / absorbRatio) * (scatterRatio / scatter) - 1),
+                                std::abs((absorb / absorbRatio) * (fissionRatio / fission) - 1),
+                                std::abs((scatter / scatterRatio) * (absorbRatio / absorb) - 1),
+                                std::abs((scatter / scatterRatio) * (fissionRatio / fission) - 1),
+                                std::abs((fission / fissionRatio) * (absorbRatio / absorb) - 1),
+                                std::abs((fission / fissionRatio) * (scatterRatio / scatter) - 1)};
+        r->ratio_pass = 1;
+        r->ratio_error = 0;
+        for (double e : errs) {
+            if (e > tol) r->ratio_pass = 0;
+            if (!(e <= r->ratio_error)) r->ratio_error = e;
+        }
+        r->ratio_tolerance = tol;
+        // BalanceEventTest (CoralBenchmark.cc:119-150)
+        const uint64_t facet = b.num_segments - b.census - b.collision;
+        r->event_ratio = std::abs((double(facet) / double(b.collision)) - 1);
+        r->event_pass = r->event_ratio > (1.0 / 100.0) ? 0 : 1;
+        // MissingParticleTest (CoralBenchmark.cc:152-174)
+        r->gains = b.start + b.source + b.produce + b.split;
+        r->losses = b.absorb + b.census + b.escape + b.rr + b.fission;
+        r->missing_pass = r->gains == r->losses ? 1 : 0;
+        // FluenceTest (CoralBenchmark.cc:177-228), one domain
+        const int nc = p->tables.nCells;
+        std::vector<double> fl(nc);
+        setDevice(p);
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        hipCheck(hipMemcpy(fl.data(), p->dp.fluence, nc * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        double sum = 0.0;
+        for (int c = 0; c < nc; c++) sum += fl[c];
+        const double average = sum / nc;
+        double maxDiff = 0.0;
+        for (int c = 0; c < nc; c++) {
+            const double v = fl[c];
+            const double pd = (((v > average) ? v - average : average - v) / ((v + average) / 2.0)) * 100;
+            maxDiff = (maxDiff > pd) ? maxDiff : pd;
+        }
+        r->fluence_max_percent_diff = maxDiff;
+        r->fluence_pass = maxDiff > 6.0 ? 0 : 1;
     })
 }
 

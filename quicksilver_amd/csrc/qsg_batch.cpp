@@ -206,6 +206,7 @@ void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& w
 // flux sums, tallies and states to the host.
 void enqueueFinal(qsg_batch_s* b, const qs::LaunchCtx& c) {
     const int n = (int)b->probs.size();
+    for (qsg_problem_s* p : b->probs) qs::launchFluence(c, p->dp);
     qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
     hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
              "hipMemcpyAsync");
@@ -329,8 +330,8 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
             continue;
         }
         const qs::DevCounters& d = b->hCounters[i];
-        if (balances) {
-            qsg_balance& bal = balances[i];
+        {
+            qsg_balance bal;
             bal.start = st.start;
             bal.source = p->tables.sourcePerCycle;
             bal.rr = d.rr;
@@ -344,6 +345,8 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
             bal.census = d.census;
             bal.num_segments = d.numSegments;
             bal.end = st.nCensus;
+            qsgi::addBalance(p->cumulative, bal);
+            if (balances) balances[i] = bal;
         }
         if (fluxSums) fluxSums[i] = b->hFluxSums[i];
         // the problem handle continues from the batch's state

@@ -32,6 +32,23 @@ struct PBuf {
 // Records an error message for qsg_last_error() and returns its code.
 int fail(const std::exception& e);
 
+// Balance::Add (Tallies.hh:84-99): every counter summed.
+inline void addBalance(qsg_balance& acc, const qsg_balance& b) {
+    acc.start += b.start;
+    acc.source += b.source;
+    acc.rr += b.rr;
+    acc.split += b.split;
+    acc.absorb += b.absorb;
+    acc.scatter += b.scatter;
+    acc.fission += b.fission;
+    acc.produce += b.produce;
+    acc.collision += b.collision;
+    acc.escape += b.escape;
+    acc.census += b.census;
+    acc.num_segments += b.num_segments;
+    acc.end += b.end;
+}
+
 // Device memory for the per-wave secondary stacks of tracking launches,
 // grown on demand; the owner's stream is synchronised before memory that a
 // launch may still use is freed.
@@ -80,6 +97,7 @@ struct qsg_problem_s {
     std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
     double lastTrackingSeconds = 0.0;
     qsg_stats stats = {};
+    qsg_balance cumulative = {};  // Tallies::_balanceCumulative (Tallies.cc:65)
 
     qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
 

@@ -6,6 +6,9 @@ Fixtures written next to this script:
                             and scalar-flux sums (%.17e, as JSON floats)
   tiny_cube_flux.npy        per-(cell, group) scalar flux of the last cycle
   tiny_cube_census.npz      census particles of the last cycle
+  coral2_p*_fluence.npy     cumulative per-cell fluence of the CORAL2 decks
+                            (their JSON also holds the reference's CORAL2
+                            PASS/FAIL verdict lines)
 
 The reference is the unmodified Quicksilver CPU path built by
 oracle/Makefile into oracle/_ref/qs_ref (run from this container, where the
@@ -33,6 +36,15 @@ FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "pro
           "collision", "escape", "census", "num_segments", "end"]
 
 
+# Input decks the reference ships (Examples/), copied verbatim as fixture
+# inputs: the CORAL2 benchmark problems at their one-rank size.
+REFERENCE_EXAMPLES = "/root/reference/Examples"
+EXAMPLE_DECKS = {
+    "coral2_p1": "CORAL2_Benchmark/Problem1/Coral2_P1_1.inp",
+    "coral2_p2": "CORAL2_Benchmark/Problem2/Coral2_P2_1.inp",
+}
+
+
 def fixture_decks():
     return {
         "small_cube": D.small_cube(),
@@ -58,8 +70,10 @@ def run_reference(deck_path: str, threads: int = 8, env_extra=None):
         env.update(env_extra)
     out = subprocess.run([QS_REF, "-i", deck_path], capture_output=True, text=True, env=env,
                          check=True, cwd=os.path.dirname(deck_path)).stdout
-    cycles, fom = [], None
+    cycles, fom, verdicts = [], None, []
     for line in out.splitlines():
+        if "PASS::" in line or "FAIL::" in line:
+            verdicts.append(line.strip())
         tok = line.split()
         if tok and tok[0] == "QSREF":
             vals = [int(t) for t in tok[2:15]]
@@ -68,26 +82,40 @@ def run_reference(deck_path: str, threads: int = 8, env_extra=None):
         elif tok and tok[0] == "QSREF_FOM":
             fom = {"segments": int(tok[1]), "tracking_seconds": float(tok[2]),
                    "segments_per_second": float(tok[3]), "threads": int(tok[4])}
-    return cycles, fom
+    return cycles, fom, verdicts
 
 
 def main(names):
     os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
     all_decks = fixture_decks()
-    for name in names or list(all_decks):
-        deck = all_decks[name]
-        path = deck.write(os.path.join(HERE, "decks", name + ".inp"))
+    for name in names or list(all_decks) + list(EXAMPLE_DECKS):
+        path = os.path.join(HERE, "decks", name + ".inp")
+        if name in EXAMPLE_DECKS:
+            with open(os.path.join(REFERENCE_EXAMPLES, EXAMPLE_DECKS[name])) as f:
+                text = f.read()
+            with open(path, "w") as f:
+                f.write(text)
+        else:
+            all_decks[name].write(path)
         env = {}
         tmp = tempfile.mkdtemp()
         if name == "tiny_cube":
             env = {"QSREF_FLUX_DUMP": os.path.join(tmp, "flux.bin"),
                    "QSREF_CENSUS_DUMP": os.path.join(tmp, "census.txt")}
-        cycles, fom = run_reference(path, env_extra=env)
+        if name in EXAMPLE_DECKS:
+            env = {"QSREF_FLUENCE_DUMP": os.path.join(tmp, "fluence.bin")}
+        cycles, fom, verdicts = run_reference(path, env_extra=env)
+        record = {"deck": f"decks/{name}.inp", "source": "oracle/_ref/qs_ref (reference CPU path)",
+                  "cycles": cycles, "fom": fom}
+        if name in EXAMPLE_DECKS:
+            record["example"] = "Examples/" + EXAMPLE_DECKS[name]
+            record["coral_verdicts"] = verdicts
+            np.save(os.path.join(HERE, name + "_fluence.npy"),
+                    np.fromfile(env["QSREF_FLUENCE_DUMP"], dtype=np.float64))
         with open(os.path.join(HERE, name + ".json"), "w") as f:
-            json.dump({"deck": f"decks/{name}.inp", "source": "oracle/_ref/qs_ref (reference CPU path)",
-                       "cycles": cycles, "fom": fom}, f, indent=1)
+            json.dump(record, f, indent=1)
         if name == "tiny_cube":
-            n_cells = deck.n_cells
+            n_cells = all_decks[name].n_cells
             flux = np.fromfile(env["QSREF_FLUX_DUMP"], dtype=np.float64).reshape(-1, n_cells, D.N_GROUPS)
             np.save(os.path.join(HERE, "tiny_cube_flux.npy"), flux[-1])
             cols = np.loadtxt(env["QSREF_CENSUS_DUMP"], dtype=str, ndmin=2)

@@ -64,7 +64,8 @@ struct DevProblem {
     double dx, dy, dz;     // lx/nx ...
     double hdx, hdy, hdz;  // dx/2.0 ...
     int boundaryMask;      // 2 bits per face +x -x +y -y +z -z: 1 escape, 2 reflect
-    const double* planes;
+    const double* planes;    // [nCells][24][A,B,C,D]
+    const double* planeND;   // [nCells][24][2]: the facet's one nonzero normal component and D
     const int32_t* cellMat;
     const double* cellVolume;
     const double* cellCenter;

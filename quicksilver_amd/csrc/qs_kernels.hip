@@ -261,6 +261,43 @@ struct FindNearest {
 };
 
 // The four facets of the face of axis AX the lane leaves through.
+// The same test for a plane whose other two normal components are zero
+// (B = C = 0 for an x face, ...): its dot product is N*dir[AX] and its
+// numerator -(N*pos[AX] + D).  Dropping the zero terms changes no value:
+// B*y and C*z are signed zeros, and adding a signed zero to a nonzero sum
+// leaves it unchanged; when the sum is itself zero only the sign of a zero
+// distance can differ, and no comparison or product below can tell.
+template <int AX>
+__device__ __forceinline__ double distance_to_face_facet(double planeTol, double dot, double N, double D,
+                                                         double p0u, double p0v, double p1u, double p1v,
+                                                         double p2u, double p2v, double x, double y, double z,
+                                                         double a, double b, double g) {
+    const double tol = 1e-9;
+    const double xa = AX == 0 ? x : (AX == 1 ? y : z);
+    double numerator = -1.0 * (N * xa + D);
+    if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
+    double distance = numerator / dot;
+    double ix = x + distance * a;
+    double iy = y + distance * b;
+    double iz = z + distance * g;
+    const double iu = AX == 0 ? iy : (AX == 1 ? iz : ix);
+    const double iv = AX == 0 ? iz : (AX == 1 ? ix : iy);
+    double iuP = iu + tol, iuM = iu - tol, ivP = iv + tol, ivM = iv - tol;
+    if (p0u > iuP && p1u > iuP && p2u > iuP) return kHuge;
+    if (p0u < iuM && p1u < iuM && p2u < iuM) return kHuge;
+    if (p0v > ivP && p1v > ivP && p2v > ivP) return kHuge;
+    if (p0v < ivM && p1v < ivM && p2v < ivM) return kHuge;
+    double cross1 = (p1u - p0u) * (iv - p0v) - (p1v - p0v) * (iu - p0u);
+    double cross2 = (p2u - p1u) * (iv - p1v) - (p2v - p1v) * (iu - p1u);
+    double cross0 = (p0u - p2u) * (iv - p2v) - (p0v - p2v) * (iu - p2u);
+    double s = cross0 + cross1 + cross2;
+    double ctol = 1e-9 * (s < 0 ? -s : s);
+    if ((cross0 > -ctol && cross1 > -ctol && cross2 > -ctol) ||
+        (cross0 < ctol && cross1 < ctol && cross2 < ctol))
+        return distance;
+    return kHuge;
+}
+
 template <int AX, int T>
 __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
                                             double x, double y, double z, double a, double b, double g, bool plus,
@@ -274,15 +311,15 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, c
         fn.add(kHuge, FM);
         return;
     }
-    const double* p = pl + 4 * f;
-    const double A = p[0], B = p[1], C = p[2], D = p[3];
-    const double dot = A * a + B * b + C * g;
+    const double* p = pl + 2 * f;
+    const double N = p[0], D = p[1];
+    const double dot = N * (AX == 0 ? a : (AX == 1 ? b : g));
     double d = kHuge;
     if (dot > 0.0) {
         const double p0u = node_coord<FP, FM, 0, U>(c9, plus), p0v = node_coord<FP, FM, 0, V>(c9, plus);
         const double p1u = node_coord<FP, FM, 1, U>(c9, plus), p1v = node_coord<FP, FM, 1, V>(c9, plus);
         const double p2u = node_coord<FP, FM, 2, U>(c9, plus), p2v = node_coord<FP, FM, 2, V>(c9, plus);
-        d = distance_to_facet<AX>(planeTol, dot, A, B, C, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
+        d = distance_to_face_facet<AX>(planeTol, dot, N, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
     }
     // the opposite face faces away from the direction: distance kHuge
     fn.add(kHuge, plus ? FM : FP);
@@ -332,16 +369,16 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
 // distance kHuge in the reference too), fully unrolled; the rare retries
 // use a compact loop over all 24 facets so they add no register pressure
 // to the common path.
-__device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell,
-                                                 const Cell9& c9, double& x, double& y, double& z,
+__device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, const double* pnd,
+                                                 int cell, const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
     Nearest nf;
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
         FindNearest fn = {1e80, -1, -1, -kHuge, 0};
-        visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
-        visit_axis<1>(fn, pl, c9, planeTol, x, y, z, a, b, g);
-        visit_axis<2>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+        visit_axis<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        visit_axis<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        visit_axis<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
         nf = fn.result();
     }
     if (nf.d == kHuge) {
@@ -583,6 +620,7 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.hdz = rfld(c.hdz);
     P.boundaryMask = rfl(c.boundaryMask);
     P.planes = rflp(c.planes);
+    P.planeND = rflp(c.planeND);
     P.cellMat = rflp(c.cellMat);
     P.cellVolume = rflp(c.cellVolume);
     P.cellCenter = rflp(c.cellCenter);
@@ -815,7 +853,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         double dCen = L.speed * L.tc;
         const Cell9 c9 = cell_nodes(P, L.ci, L.cj, L.ck);
         const double* pl = P.planes + (size_t)cell * 96;
-        Nearest nf = nearest_facet(P, pl, cell, c9, L.x, L.y, L.z, L.a, L.b, L.g);
+        Nearest nf = nearest_facet(P, pl, P.planeND + (size_t)cell * 48, cell, c9, L.x, L.y, L.z, L.a, L.b, L.g);
         double dFacet = nf.d;
         if (force) {
             dFacet = kHuge;
@@ -855,12 +893,44 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             double r = rng_sample(L.seed);
             double cur = sigt * r;
             int selR = -1;
-            for (int iso = 0; iso < mi.nIso && cur >= 0; iso++) {
-                for (int rr = 0; rr < mi.nReact; rr++) {
-                    cur -= T.macro[(mat * P.maxR + rr) * G + L.group];
+            // Every isotope of a material carries the same per-reaction
+            // macroscopic cross sections, so the scan subtracts the same
+            // nReact values once per isotope; with up to three reactions
+            // they are read once and held in registers.
+            const double* mrow = T.macro + (size_t)(mat * P.maxR) * G + L.group;
+            if (mi.nReact <= 3) {
+                const double m0 = mrow[0];
+                const double m1 = mi.nReact > 1 ? mrow[G] : 0.0;
+                const double m2 = mi.nReact > 2 ? mrow[2 * G] : 0.0;
+                for (int iso = 0; iso < mi.nIso; iso++) {
+                    cur -= m0;
                     if (cur < 0) {
-                        selR = rr;
+                        selR = 0;
                         break;
+                    }
+                    if (mi.nReact > 1) {
+                        cur -= m1;
+                        if (cur < 0) {
+                            selR = 1;
+                            break;
+                        }
+                    }
+                    if (mi.nReact > 2) {
+                        cur -= m2;
+                        if (cur < 0) {
+                            selR = 2;
+                            break;
+                        }
+                    }
+                }
+            } else {
+                for (int iso = 0; iso < mi.nIso && cur >= 0; iso++) {
+                    for (int rr = 0; rr < mi.nReact; rr++) {
+                        cur -= mrow[(size_t)rr * G];
+                        if (cur < 0) {
+                            selR = rr;
+                            break;
+                        }
                     }
                 }
             }

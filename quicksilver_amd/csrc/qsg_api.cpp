@@ -129,6 +129,20 @@ void build(qsg_problem_s* p, int device) {
     d.boundaryMask = 0;
     for (int f = 0; f < 6; f++) d.boundaryMask |= (T.boundary[f] & 3) << (2 * f);
     d.planes = p->upload(T.planes);
+    {
+        // The planes are axis-aligned (checked when the tables are built):
+        // facet f of a cell has B = C = 0 (x faces, f < 8), A = C = 0 (y) or
+        // A = B = 0 (z).  The tracking loop reads only the nonzero component
+        // and D, in a compact copy.
+        std::vector<double> nd((size_t)T.nCells * 24 * 2);
+        for (size_t c = 0; c < (size_t)T.nCells; c++)
+            for (int f = 0; f < 24; f++) {
+                const double* pl = &T.planes[(c * 24 + f) * 4];
+                nd[(c * 24 + f) * 2 + 0] = pl[f / 8];
+                nd[(c * 24 + f) * 2 + 1] = pl[3];
+            }
+        d.planeND = p->upload(nd);
+    }
     d.cellMat = p->upload(T.cellMaterial);
     d.cellVolume = p->upload(T.cellVolume);
     d.cellCenter = p->upload(T.cellCenter);

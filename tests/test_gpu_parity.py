@@ -119,3 +119,4 @@ def test_device_fp64_builtins_round_like_ieee():
         h = fn(x)
         ulp = np.spacing(np.abs(h))
         assert np.max(np.abs(d - h) / ulp) <= 2.0
+

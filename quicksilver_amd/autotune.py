@@ -41,7 +41,8 @@ SPACE = {
     "sort_interval": [0, 1, 2, 4],
     "waves_per_simd": [2, 3, 4],
 }
-ORDER = ["waves_per_simd", "waves_per_cu", "refill_min", "chunk", "block_size", "stack_entries"]
+ORDER = ["waves_per_simd", "waves_per_cu", "refill_min", "chunk", "block_size", "stack_entries",
+         "sort_interval"]
 DEFAULT = dict(q.DEFAULT_TUNING)
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 

@@ -146,7 +146,7 @@ struct BatchState {
     uint32_t cycle;     // cycles completed; processing buffer = proc[cycle & 1]
     uint32_t active;    // takes part in the next step
     uint32_t mode;      // population control: 0 none, 1 roulette, 2 split
-    uint32_t pad;
+    uint32_t sorted;    // this cycle's queue was sorted into proc[cycle & 1]
     double factor;      // population-control factor of the current cycle
 };
 
@@ -214,6 +214,24 @@ void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint
 void launchBatchPopulation(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* popPrefix);
 // passKind 0: pop -> extra[0]; 1: extra[0] -> extra[1]; 2: extra[1] -> extra[0]
 void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind);
+
+// ---- particle sort by (cell, energy band) ----
+// The tracking queue is reordered so that neighbouring lanes start in the
+// same cell and energy band (counting sort; order within a key is
+// arbitrary, which no tally depends on).  keys = nCells * kSortBands.
+constexpr int kSortBands = 4;
+inline size_t sortKeys(int nCells) { return (size_t)nCells * kSortBands; }
+inline size_t scanBlocks(size_t m) { return (m + 1023) / 1024; }
+// Plain form: sorts in[0, *n) into out; counts has sortKeys(nCells)
+// entries, blockSums scanBlocks(keys).
+void launchSort(const LaunchCtx& ctx, const DevProblem& P, Particles in, const uint32_t* n, Particles out,
+                uint32_t* counts, uint32_t* blockSums);
+// Batched form: every problem whose cycle is a multiple of `interval` sorts
+// its population-control output into proc[cycle & 1] and marks the state
+// sorted.  keyBase: [n + 1] exclusive prefix of the problems' key counts;
+// sortPrefix: [n + 1] scratch.
+void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval, const uint64_t* keyBase,
+                     uint64_t* sortPrefix, uint32_t* counts, size_t totalKeys, uint32_t* blockSums);
 void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
                       const WaveStacks& ws);
 void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,

@@ -682,7 +682,8 @@ struct BatchQueue {
         const int outIdx = passKind == 1 ? 1 : 0;
         const uint32_t cyc = (uint32_t)rfl((int)st->cycle);
         const uint32_t cap = (uint32_t)rfl((int)S.cap);
-        QueueIO io = {rflv(passKind == 0 ? S.pop : S.extra[passKind == 1 ? 0 : 1]),
+        const bool sorted = rfl((int)st->sorted) != 0;
+        QueueIO io = {rflv(passKind == 0 ? (sorted ? S.proc[cyc & 1] : S.pop) : S.extra[passKind == 1 ? 0 : 1]),
                       (uint32_t)rfl((int)*inCount(p)),
                       &st->head,
                       rflv(S.proc[(cyc + 1) & 1]),
@@ -1423,6 +1424,98 @@ __global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux
     }
 }
 
+// ------------------------------------------------------- particle sort ----
+// Counting sort of a tracking queue by (cell, energy band): histogram of
+// keys, exclusive scan, scatter.  The band of a particle is its energy group
+// (the same search MC_Load_Particle does) scaled to kSortBands.
+
+__device__ __forceinline__ uint32_t sort_key(const DevProblem& P, const Particles& in, uint32_t i) {
+    const int g = energy_group(P.energies, P.G, in.E()[i]);
+    return (uint32_t)in.cell()[i] * kSortBands + (uint32_t)(g * kSortBands / P.G);
+}
+
+__device__ __forceinline__ void copy_particle(const Particles& in, uint32_t i, const Particles& out, uint32_t o) {
+#pragma unroll
+    for (int k = 0; k < 10; k++) out.f(k)[o] = in.f(k)[i];
+    out.seed()[o] = in.seed()[i];
+    out.cell()[o] = in.cell()[i];
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(DevProblem P, Particles in, const uint32_t* __restrict__ nPtr,
+                                                    uint32_t* __restrict__ counts) {
+    const uint32_t n = *nPtr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&counts[sort_key(P, in, i)], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_sort_scatter(DevProblem P, Particles in, const uint32_t* __restrict__ nPtr,
+                                                      uint32_t* __restrict__ offsets, Particles out) {
+    const uint32_t n = *nPtr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t o = atomicAdd(&offsets[sort_key(P, in, i)], 1u);
+        copy_particle(in, i, out, o);
+    }
+}
+
+// Exclusive scan of m u32 values in place: per 1024-element block, then
+// the block sums (one workgroup), then the block offsets added back.
+__global__ void __launch_bounds__(256) k_scan_local(uint32_t* __restrict__ data, size_t m,
+                                                    uint32_t* __restrict__ blockSums) {
+    __shared__ uint32_t sh[256];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x * 4;
+    uint32_t v[4];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[k] = base + k < m ? data[base + k] : 0u;
+        sum += v[k];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = sh[threadIdx.x] - sum;  // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (base + k < m) data[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) blockSums[blockIdx.x] = sh[255];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ blockSums, int nb) {
+    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int q = c0 + (int)threadIdx.x;
+        const uint32_t v = q < nb ? blockSums[q] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (q < nb) blockSums[q] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += sh[1023];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t* __restrict__ data, size_t m,
+                                                  const uint32_t* __restrict__ blockSums) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+        data[i] += blockSums[i / 1024];
+}
+
 // ------------------------------------------------------ batched form ----
 // Many independent problems advance one cycle together.  Every problem keeps
 // its own particle buffers, tallies and RNG streams, so its results are the
@@ -1453,6 +1546,7 @@ __global__ void k_batch_prep(BatchSlot* slots, int n, uint64_t* srcPrefix, uint6
         if (factor != 1.0 && nProc != 0) mode = (factor < 1) ? 1u : ((factor > 1) ? 2u : 0u);
         st->factor = factor;
         st->mode = mode;
+        st->sorted = 0;
         st->head = 0;
         st->nPop = 0;
         st->extra[0] = 0;
@@ -1542,6 +1636,48 @@ __global__ void k_batch_pass_prep(BatchSlot* slots, int n, int passKind) {
         BatchState* st = slots[p].st;
         st->head = 0;
         st->extra[passKind == 1 ? 1 : 0] = 0;
+    }
+}
+
+// Batched sort: which problems sort this cycle, and the prefix of their
+// queue lengths (a problem's keys are contiguous in the global key space,
+// so the scan of its keys starts at the sum of the earlier sorted queues).
+__global__ void k_batch_sort_prep(BatchSlot* slots, int n, int interval, uint64_t* sortPrefix) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t a = 0;
+    for (int p = 0; p < n; p++) {
+        BatchState* st = slots[p].st;
+        const bool due = st->active && st->error == 0 && interval > 0 && (st->cycle % (uint32_t)interval) == 0;
+        st->sorted = due ? 1u : 0u;
+        sortPrefix[p] = a;
+        if (due) a += st->nPop;
+    }
+    sortPrefix[n] = a;
+}
+
+__global__ void __launch_bounds__(256) k_batch_sort_count(BatchSlot* slots, int n, const uint64_t* sortPrefix,
+                                                          const uint64_t* keyBase, uint32_t* counts) {
+    const uint64_t total = sortPrefix[n];
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int p = find_problem(sortPrefix, n, t);
+        const BatchSlot& S = slots[p];
+        const uint32_t i = (uint32_t)(t - sortPrefix[p]);
+        atomicAdd(&counts[keyBase[p] + sort_key(S.prob, S.pop, i)], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_sort_scatter(BatchSlot* slots, int n, const uint64_t* sortPrefix,
+                                                            const uint64_t* keyBase, uint32_t* offsets) {
+    const uint64_t total = sortPrefix[n];
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int p = find_problem(sortPrefix, n, t);
+        const BatchSlot& S = slots[p];
+        const uint32_t i = (uint32_t)(t - sortPrefix[p]);
+        const uint32_t o = atomicAdd(&offsets[keyBase[p] + sort_key(S.prob, S.pop, i)], 1u) -
+                           (uint32_t)sortPrefix[p];
+        copy_particle(S.pop, i, S.proc[S.st->cycle & 1], o);
     }
 }
 
@@ -1761,6 +1897,40 @@ void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* 
     unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_batch_flux_final, dim3(blocks), dim3(256), 0, s, slots, n, partsPerProblem, partials,
                        fluxSums);
+}
+
+namespace {
+void launchScan(hipStream_t s, uint32_t* data, size_t m, uint32_t* blockSums) {
+    const size_t nb = scanBlocks(m);
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(256), 0, s, data, m, blockSums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, blockSums, (int)nb);
+    unsigned blocks = (unsigned)((m + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks), dim3(256), 0, s, data, m, blockSums);
+}
+}  // namespace
+
+void launchSort(const LaunchCtx& ctx, const DevProblem& P, Particles in, const uint32_t* n, Particles out,
+                uint32_t* counts, uint32_t* blockSums) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    const size_t keys = sortKeys(P.nCells);
+    launchClear(ctx, counts, keys * sizeof(uint32_t));
+    const unsigned grid = (unsigned)ctx.numCUs * 8;
+    hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, s, P, in, n, counts);
+    launchScan(s, counts, keys, blockSums);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, P, in, n, counts, out);
+}
+
+void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval, const uint64_t* keyBase,
+                     uint64_t* sortPrefix, uint32_t* counts, size_t totalKeys, uint32_t* blockSums) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    launchClear(ctx, counts, totalKeys * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_batch_sort_prep, dim3(1), dim3(64), 0, s, slots, n, interval, sortPrefix);
+    const unsigned grid = (unsigned)ctx.numCUs * 8;
+    hipLaunchKernelGGL(k_batch_sort_count, dim3(grid), dim3(256), 0, s, slots, n, sortPrefix, keyBase, counts);
+    launchScan(s, counts, totalKeys, blockSums);
+    hipLaunchKernelGGL(k_batch_sort_scatter, dim3(grid), dim3(256), 0, s, slots, n, sortPrefix, keyBase, counts);
 }
 
 void launchFluence(const LaunchCtx& ctx, const DevProblem& P) {

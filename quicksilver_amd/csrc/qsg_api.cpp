@@ -176,6 +176,8 @@ void build(qsg_problem_s* p, int device) {
     d.lowWeightCutoff = T.lowWeightCutoff;
     d.flux = p->dalloc<double>((size_t)T.nCells * T.nGroups);
     hipCheck(hipMemset(d.flux, 0, (size_t)T.nCells * T.nGroups * sizeof(double)), "hipMemset");
+    p->sortCounts = p->dalloc<uint32_t>(qs::sortKeys(T.nCells));
+    p->sortBlockSums = p->dalloc<uint32_t>(qs::scanBlocks(qs::sortKeys(T.nCells)));
     d.fluence = nullptr;
     if (p->params.simulationParams.coralBenchmark) {
         d.fluence = p->dalloc<double>((size_t)T.nCells);
@@ -241,6 +243,12 @@ void cycleInit(qsg_problem_s* p) {
     int mode = 0;
     if (factor != 1.0 && n != 0) mode = (factor < 1) ? 1 : ((factor > 1) ? 2 : 0);
     qs::launchPopulation(c, p->dp, proc.v, (uint32_t)n, p->pop.v, p->pop.cap, mode, factor, p->pc, p->dc);
+    // every sort_interval cycles the queue is sorted by (cell, energy band)
+    // into this cycle's processing buffer, whose particles the population
+    // control has already consumed
+    p->queueSorted = p->tune.sortInterval > 0 && (p->cycle % p->tune.sortInterval) == 0;
+    if (p->queueSorted)
+        qs::launchSort(c, p->dp, p->pop.v, &p->pc->nPopOut, proc.v, p->sortCounts, p->sortBlockSums);
     hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
     syncCounters(p);
     checkErrors(p);
@@ -262,7 +270,7 @@ void cycleTracking(qsg_problem_s* p) {
     hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
     const uint32_t* nIn = &p->pc->nPopOut;
     uint32_t nInHost = p->nPop;
-    qs::Particles in = p->pop.v;
+    qs::Particles in = p->queueSorted ? p->proc[p->cur].v : p->pop.v;
     int k = 0;
     while (nInHost > 0) {
         uint32_t* nOut = &p->pc->extraCount[k & 1];
@@ -435,6 +443,7 @@ uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T) {
     b += (G + 1) * 8 + M * G * 8 + M * R * G * 8 + M * R * 4;
     const uint64_t cap = particleCapacity(T);
     b += 5 * (cap * qs::kParticleBytes + 12 * 256);
+    b += nc * 24 * 16 + qs::sortKeys((int)nc) * 4 + qs::scanBlocks(qs::sortKeys((int)nc)) * 4;
     return b + 64 * 1024;
 }
 
@@ -601,6 +610,7 @@ int qsg_set_particles(qsg_problem p, const qsg_particles* in, uint64_t n) {
                 throw QsgError(QSG_ERR_ARG, "particle cell id out of range");
         setDevice(p);
         copyParticles(p->pop.v, in, n, hipMemcpyHostToDevice, p->stream);
+        p->queueSorted = false;
         uint32_t n32 = (uint32_t)n;
         hipCheck(hipMemcpy(&p->pc->nPopOut, &n32, sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy");
         p->nPop = n32;
@@ -637,7 +647,7 @@ int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n) {
         if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "processing population exists after cycle_init");
         if (n > p->nPop) throw QsgError(QSG_ERR_ARG, "n exceeds the processing count");
         setDevice(p);
-        copyParticles(p->pop.v, out, n, hipMemcpyDeviceToHost, p->stream);
+        copyParticles(p->queueSorted ? p->proc[p->cur].v : p->pop.v, out, n, hipMemcpyDeviceToHost, p->stream);
     })
 }
 

@@ -21,6 +21,11 @@ struct qsg_batch_s {
     qs::DevCounters* dCounters = nullptr;
     uint64_t* dSrcPrefix = nullptr;
     uint64_t* dPopPrefix = nullptr;
+    uint64_t* dKeyBase = nullptr;     // [n + 1] prefix of the problems' sort keys
+    uint64_t* dSortPrefix = nullptr;  // [n + 1]
+    uint32_t* dSortCounts = nullptr;
+    uint32_t* dSortBlockSums = nullptr;
+    size_t sortKeys = 0;
     double* dPartials = nullptr;
     double* dFluxSums = nullptr;
     int partsPerProblem = 16;
@@ -64,7 +69,8 @@ void destroyBatch(qsg_batch_s* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void* d : {(void*)b->dSlots, (void*)b->dState, (void*)b->dCounters, (void*)b->dSrcPrefix,
-                    (void*)b->dPopPrefix, (void*)b->dPartials, (void*)b->dFluxSums})
+                    (void*)b->dPopPrefix, (void*)b->dPartials, (void*)b->dFluxSums, (void*)b->dKeyBase,
+                    (void*)b->dSortPrefix, (void*)b->dSortCounts, (void*)b->dSortBlockSums})
         if (d) (void)hipFree(d);
     for (void* h : {(void*)b->hState, (void*)b->hCounters, (void*)b->hFluxSums})
         if (h) (void)hipHostFree(h);
@@ -113,6 +119,17 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
         b->dSrcPrefix = devAlloc<uint64_t>(n + 1);
         b->dPopPrefix = devAlloc<uint64_t>(n + 1);
         b->dPartials = devAlloc<double>((size_t)n * b->partsPerProblem);
+        {
+            std::vector<uint64_t> keyBase(n + 1, 0);
+            for (int i = 0; i < n; i++) keyBase[i + 1] = keyBase[i] + qs::sortKeys(b->probs[i]->tables.nCells);
+            b->sortKeys = keyBase[n];
+            b->dKeyBase = devAlloc<uint64_t>(n + 1);
+            hipCheck(hipMemcpy(b->dKeyBase, keyBase.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice),
+                     "hipMemcpy");
+            b->dSortPrefix = devAlloc<uint64_t>(n + 1);
+            b->dSortCounts = devAlloc<uint32_t>(b->sortKeys);
+            b->dSortBlockSums = devAlloc<uint32_t>(qs::scanBlocks(b->sortKeys));
+        }
         b->dFluxSums = devAlloc<double>(n);
         b->hState = hostAlloc<qs::BatchState>(n);
         b->hCounters = hostAlloc<qs::DevCounters>(n);
@@ -193,6 +210,9 @@ void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& w
     qs::launchBatchPrep(c, b->dSlots, n, b->dSrcPrefix, b->dPopPrefix);
     qs::launchBatchSource(c, b->dSlots, n, b->dSrcPrefix);
     qs::launchBatchPopulation(c, b->dSlots, n, b->dPopPrefix);
+    if (b->tune.sortInterval > 0)
+        qs::launchBatchSort(c, b->dSlots, n, b->tune.sortInterval, b->dKeyBase, b->dSortPrefix, b->dSortCounts,
+                            b->sortKeys, b->dSortBlockSums);
     qs::launchBatchPassPrep(c, b->dSlots, n, 0);
     if (timed) hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
     qs::launchBatchTrack(c, b->dSlots, n, 0, b->tune, ws);
@@ -238,7 +258,8 @@ hipGraphExec_t capture(qsg_batch_s* b, F enqueue) {
 
 bool sameShape(const qs::TrackTuning& a, const qs::TrackTuning& t) {
     return a.blockSize == t.blockSize && a.wavesPerCU == t.wavesPerCU && a.refillMin == t.refillMin &&
-           a.chunk == t.chunk && a.stackCap == t.stackCap && a.wavesPerSimd == t.wavesPerSimd;
+           a.chunk == t.chunk && a.stackCap == t.stackCap && a.wavesPerSimd == t.wavesPerSimd &&
+           a.sortInterval == t.sortInterval;
 }
 
 // The step's two graphs, captured again whenever the launch shape or the

@@ -82,6 +82,10 @@ struct qsg_problem_s {
     int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
     int cycle = 0;
 
+    uint32_t* sortCounts = nullptr;     // [nCells * kSortBands]
+    uint32_t* sortBlockSums = nullptr;  // scan scratch
+    bool queueSorted = false;           // this cycle's queue is proc[cur] (sorted), not pop
+
     qs::PassCounters* pc = nullptr;
     qs::DevCounters* dc = nullptr;
     qs::PassCounters* hpc = nullptr;  // pinned host mirrors

@@ -25,9 +25,11 @@ def _check(name, g, got):
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape", [{}, {"graphs": 0}, {"waves_per_simd": 2},
                                    {"waves_per_simd": 4, "stack_entries": 0},
-                                   {"block_size": 64, "waves_per_cu": 3, "chunk": 8}],
+                                   {"block_size": 64, "waves_per_cu": 3, "chunk": 8},
+                                   {"sort_interval": 1}, {"sort_interval": 2, "graphs": 0}],
                          ids=["default_graphs", "eager_launches", "registers_2_waves",
-                              "registers_4_waves_passes_only", "few_waves"])
+                              "registers_4_waves_passes_only", "few_waves", "sorted_graphs",
+                              "sorted_every_2nd_eager"])
 def test_mixed_batch_matches_reference_per_problem(shape):
     print(f"[batch] mixed fixture decks, tuning {shape}", flush=True)
     golds = [load_golden(n) for n in MIX]

@@ -8,8 +8,8 @@ seconds, so the checks are the size-independent ones:
   pinned on every reference fixture in test_invariants.py);
 * the tallies do not depend on how the work is executed: the first cycles
   run with secondaries through extra passes instead of wave stacks, with a
-  different register budget, and inside a batch, give bit-identical balance
-  tallies and flux within 1e-10;
+  different register budget, with the queue sorted by cell, and inside a
+  batch, give bit-identical balance tallies and flux within 1e-10;
 * the same generator at 40^3 (checker_small) is bit-exact against the
   reference (test_gpu_parity.py).
 """
@@ -49,7 +49,8 @@ def test_checkerboard_80_execution_order_does_not_change_tallies(deck_text):
     print("[large] execution-order invariance, 2 cycles", flush=True)
     runs = {}
     for label, shape in [("default", {}), ("passes_2_waves", {"stack_entries": 0, "waves_per_simd": 2}),
-                         ("small_grid", {"block_size": 128, "waves_per_cu": 4, "chunk": 256})]:
+                         ("small_grid", {"block_size": 128, "waves_per_cu": 4, "chunk": 256}),
+                         ("sorted", {"sort_interval": 1})]:
         with q.Problem(deck_text) as p:
             if shape:
                 p.set_tuning(**shape)

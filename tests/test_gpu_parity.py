@@ -58,6 +58,8 @@ SHAPES = {
     "coarse_chunks": {"block_size": 128, "waves_per_cu": 32, "refill_min": 32, "chunk": 1024},
     "registers_2_waves": {"waves_per_simd": 2},
     "registers_4_waves": {"waves_per_simd": 4, "waves_per_cu": 16},
+    "sorted_every_cycle": {"sort_interval": 1},
+    "sorted_every_2nd_passes_only": {"sort_interval": 2, "stack_entries": 0},
 }
 
 

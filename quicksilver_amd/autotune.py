@@ -13,14 +13,19 @@ Knobs (qsg_tuning):
 
 Method: coordinate descent from the current best.  Each knob in turn is swept
 over its candidate values with the others fixed, scoring every candidate by
-the tracking-kernel throughput (segments per second of summed kernel time,
-HIP events on the library's stream) over a few cycles of the workload
-itself.  The sweep repeats until a full round changes nothing (at most
-``rounds`` rounds).  Results are cached in a JSON table keyed by a workload
-signature, so later runs start from the tuned shape.
+segments per second of device time (HIP events on the library's stream)
+over a few cycles of the workload itself: tracking plus cycle set-up for
+one problem (so a sort pays for itself), the captured first half of each
+step for a batch.  Each candidate runs on a fresh copy of the workload
+over the same cycles, so every candidate does identical work (the tallies
+do not depend on the shape).  The sweep repeats until a full round changes
+nothing (at most ``rounds`` rounds).  Results are cached in a JSON table
+keyed by a workload signature, so later runs start from the tuned shape.
 
 Usage:
-    python -m quicksilver_amd.autotune deck.inp [--batch deck2.inp ...]
+    python -m quicksilver_amd.autotune deck.inp                 (plain form)
+    python -m quicksilver_amd.autotune --batch a.inp b.inp ...  (batched form)
+    python -m quicksilver_amd.autotune --mixed-batch 64         (the 64-problem mix)
 """
 from __future__ import annotations
 
@@ -112,50 +117,69 @@ def coordinate_descent(score: Callable[[Shape], float], start: Optional[Shape] =
 
 def tune_plain(deck_text: str, warmup: int = 2, cycles: int = 2, rounds: int = 2,
                log: Callable[[str], None] = print) -> Tuple[Shape, float]:
-    with q.Problem(deck_text) as p:
-        p.run(warmup)
+    """Every candidate runs on a fresh copy of the problem over the same
+    cycles (warm-up, then `cycles` timed), so all candidates do identical
+    work.  Score: segments per second of device time in cycle set-up
+    (source, population control, sort) plus tracking."""
 
-        def score(s: Shape) -> float:
+    def score(s: Shape) -> float:
+        with q.Problem(deck_text) as p:
             p.set_tuning(**s)
+            p.run(warmup)
             p.reset_stats()
             p.run(cycles)
             st = p.stats()
-            return st["segments"] / st["track_kernel_seconds"] if st["track_kernel_seconds"] > 0 else 0.0
+        t = st["track_kernel_seconds"] + st["init_seconds"]
+        return st["segments"] / t if t > 0 else 0.0
 
-        return coordinate_descent(score, lookup(signature("plain", [deck_text])), rounds, log)
+    start = lookup(signature("plain", [deck_text])) or dict(DEFAULT)
+    return coordinate_descent(score, start, rounds, log)
 
 
-def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 1, rounds: int = 1,
+def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 2, rounds: int = 1,
                log: Callable[[str], None] = print) -> Tuple[Shape, float]:
-    probs = [q.Problem(t) for t in deck_texts]
-    try:
-        with q.Batch(probs) as b:
-            for _ in range(warmup):
-                b.step()
+    """As tune_plain, on a fresh batch of the decks per candidate.  Score:
+    segments per second of the timed steps' captured first halves (set-up,
+    source, population control, sort and the tracking launch)."""
 
-            def score(s: Shape) -> float:
+    def score(s: Shape) -> float:
+        probs = [q.Problem(t) for t in deck_texts]
+        try:
+            with q.Batch(probs) as b:
                 b.set_tuning(**s)
+                for _ in range(warmup):
+                    b.step()
                 st0 = b.stats()
                 for _ in range(cycles):
                     b.step()
                 st = b.stats()
-                dt = st["track_kernel_seconds"] - st0["track_kernel_seconds"]
-                return (st["segments"] - st0["segments"]) / dt if dt > 0 else 0.0
+        finally:
+            for p in probs:
+                p.close()
+        dt = st["track_kernel_seconds"] - st0["track_kernel_seconds"]
+        return (st["segments"] - st0["segments"]) / dt if dt > 0 else 0.0
 
-            return coordinate_descent(score, lookup(signature("batch", deck_texts)), rounds, log)
-    finally:
-        for p in probs:
-            p.close()
+    start = lookup(signature("batch", deck_texts)) or dict(DEFAULT)
+    return coordinate_descent(score, start, rounds, log)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("decks", nargs="+")
+    ap.add_argument("decks", nargs="*")
     ap.add_argument("--batch", action="store_true", help="tune the batched form over all decks")
+    ap.add_argument("--mixed-batch", type=int, default=0, metavar="N",
+                    help="tune the batched form on the first N mixed_batch decks (quicksilver_amd.decks)")
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--no-store", action="store_true")
     a = ap.parse_args()
-    texts = [open(d).read() for d in a.decks]
+    if a.mixed_batch:
+        from quicksilver_amd import decks as D
+        texts = [d.text() for d in D.mixed_batch(count=a.mixed_batch)]
+        a.batch = True
+    elif a.decks:
+        texts = [open(d).read() for d in a.decks]
+    else:
+        ap.error("give deck files or --mixed-batch N")
     if a.batch:
         shape, sc = tune_batch(texts, rounds=a.rounds)
         sig = signature("batch", texts)

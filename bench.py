@@ -97,7 +97,64 @@ def cpu_baseline(steps: int):
     return None
 
 
-def batched_vs_plain(n_problems: int):
+def run_reference(deck_text: str):
+    """Runs the reference CPU path (oracle/_ref/qs_ref, OpenMP on this host)
+    on one deck; returns its per-cycle balances, segments and tracking
+    seconds."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fields = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce", "collision",
+              "escape", "census", "num_segments", "end"]
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = os.path.join(tmp, "deck.inp")
+        with open(deck, "w") as f:
+            f.write(deck_text)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        out = subprocess.run([QS_REF, "-i", deck], capture_output=True, text=True, env=env, cwd=tmp,
+                             timeout=600, check=True).stdout
+    bals, seg, secs = [], 0, 0.0
+    for line in out.splitlines():
+        tok = line.split()
+        if tok and tok[0] == "QSREF":
+            bals.append(dict(zip(fields, (int(t) for t in tok[2:15]))))
+        elif tok and tok[0] == "QSREF_FOM":
+            seg, secs = int(tok[1]), float(tok[2])
+    return bals, seg, secs, threads
+
+
+def cpu_batch_sample(ds, gpu_balances, budget_segments: float = 1.2e8):
+    """The reference CPU path on a bounded sample of the mix (decks in order,
+    each taken if its segments still fit the budget): segments/s over the sum
+    of their tracking times, and whether every balance tally equals the
+    GPU's."""
+    if not os.path.exists(QS_REF):
+        return None
+    seg = 0
+    secs = 0.0
+    used = 0
+    same = True
+    threads = 0
+    names = []
+    for i, (d, gb) in enumerate(zip(ds, gpu_balances)):
+        if seg + sum(b["num_segments"] for b in gb) > budget_segments:
+            continue
+        bals, s, t, threads = run_reference(d.text())
+        same = same and bals == gb
+        seg += s
+        secs += t
+        used += 1
+        names.append(i)
+    if used == 0:
+        return None
+    return {"value": seg / secs if secs > 0 else None, "unit": "segments/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{used} of the {len(ds)} decks (indices {names}: taken in order while their GPU "
+                      f"segment counts fit {budget_segments:.2g} segments), all their steps, run serially; "
+                      "segments / sum of the reference's tracking times (its FOM; excludes source and "
+                      "population control)",
+            "segments": seg, "tallies_identical_to_gpu": same}
+
+
+def batched_vs_plain(n_problems: int, with_cpu: bool):
     """mixed_batch workload: the same decks one at a time vs in one batch;
     per-problem tallies must be identical."""
     from quicksilver_amd import autotune, decks, workloads
@@ -132,6 +189,9 @@ def batched_vs_plain(n_problems: int):
     }
     if not same:
         raise SystemExit("batched tallies differ from one-at-a-time tallies")
+    if with_cpu:
+        progress("mixed_batch: reference CPU path on a sample of the decks")
+        out["cpu_reference"] = cpu_batch_sample(ds, batch["balances"])
     return out, n_problems / batch["wall_s"]
 
 
@@ -183,7 +243,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (mixed_batch) comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
     ap.add_argument("--batch-problems", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -263,7 +323,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_batch:
-        line["batched"], capacity = batched_vs_plain(args.batch_problems)
+        line["batched"], capacity = batched_vs_plain(args.batch_problems, not args.no_cpu_baseline)
         if not args.no_serve:
             line["serving"] = serving(args.serve_requests, capacity)
     if not args.no_cpu_baseline:

@@ -164,7 +164,7 @@ int qsg_batch_get_stats(qsg_batch b, qsg_stats* stats);
  * Solve requests (deck text) are queued, admitted in arrival order while their
  * device footprint fits hbm_budget_bytes (and at most max_active run, 0 = no
  * cap), kept resident on the device, advanced together one cycle per
- * qsg_server_step (a batch rebuilt whenever membership changes), and retired
+ * qsg_server_step (requests take and free slots of one running batch), and retired
  * after their nSteps cycles.  A request's per-cycle tallies equal those of the
  * same deck run alone.  No reference counterpart.
  * Request states: 0 queued, 1 running, 2 done, 3 failed. */

@@ -207,6 +207,8 @@ void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
 // fluence[cell] += flux[cell][g] for g = 0..G-1 in order (Fluence::compute,
 // Tallies.cc:100-120); call before the flux is reduced and zeroed.
 void launchFluence(const LaunchCtx& ctx, const DevProblem& P);
+// The same for every active problem of a batch that accumulates fluence.
+void launchBatchFluence(const LaunchCtx& ctx, BatchSlot* slots, int n);
 
 // Batched launchers over `n` slots.  prefix arrays hold n+1 entries.
 void launchBatchPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, uint64_t* srcPrefix, uint64_t* popPrefix);

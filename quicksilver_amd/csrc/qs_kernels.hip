@@ -1697,6 +1697,20 @@ k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune, WaveStack
     run_queue(acc, tune, ws);
 }
 
+// Fluence of every active problem that accumulates it (one block row per
+// problem, grid-stride over its cells).
+__global__ void __launch_bounds__(256) k_batch_fluence(BatchSlot* slots) {
+    const BatchSlot& S = slots[blockIdx.y];
+    if (!S.st->active || !S.prob.fluence) return;
+    const int G = S.prob.G;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < S.prob.nCells; c += gridDim.x * blockDim.x) {
+        double v = S.prob.fluence[c];
+        const double* f = S.prob.flux + (size_t)c * G;
+        for (int g = 0; g < G; g++) v += f[g];
+        S.prob.fluence[c] = v;
+    }
+}
+
 // Flux sum per problem in a fixed order (Tallies.cc:146-171), flux reset
 // (Tallies.cc:95), and the cycle advance of every active problem.
 __global__ void __launch_bounds__(kFluxBlock) k_batch_flux_partial(BatchSlot* slots, int partsPerProblem,
@@ -1931,6 +1945,10 @@ void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval
     hipLaunchKernelGGL(k_batch_sort_count, dim3(grid), dim3(256), 0, s, slots, n, sortPrefix, keyBase, counts);
     launchScan(s, counts, totalKeys, blockSums);
     hipLaunchKernelGGL(k_batch_sort_scatter, dim3(grid), dim3(256), 0, s, slots, n, sortPrefix, keyBase, counts);
+}
+
+void launchBatchFluence(const LaunchCtx& ctx, BatchSlot* slots, int n) {
+    hipLaunchKernelGGL(k_batch_fluence, dim3(64, (unsigned)n), dim3(256), 0, (hipStream_t)ctx.stream, slots);
 }
 
 void launchFluence(const LaunchCtx& ctx, const DevProblem& P) {

@@ -26,6 +26,8 @@ struct qsg_batch_s {
     uint32_t* dSortCounts = nullptr;
     uint32_t* dSortBlockSums = nullptr;
     size_t sortKeys = 0;
+    size_t sortKeysCap = 0;
+    bool graphsValid = false;  // cleared when memory the graphs hold is reallocated
     double* dPartials = nullptr;
     double* dFluxSums = nullptr;
     int partsPerProblem = 16;
@@ -84,26 +86,87 @@ void destroyBatch(qsg_batch_s* b) {
     delete b;
 }
 
-qsg_batch_s* createBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
-    if (n <= 0 || !problems) throw QsgError(QSG_ERR_ARG, "a batch needs at least one problem");
+void checkMember(const qsg_batch_s* b, const qsg_problem_s* p) {
+    if (p->device != b->device) throw QsgError(QSG_ERR_ARG, "batch problems must share one device");
+    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "batch problems must be at a cycle boundary");
+    if (p->cur != (p->cycle & 1)) throw QsgError(QSG_ERR_STATE, "problem buffer parity does not match its cycle");
+}
+
+// Sort-key layout of the current members: keyBase[i] is the first key of
+// slot i (empty slots have none).  Grows the key histogram when needed,
+// which invalidates captured graphs (they hold its address).
+void uploadKeyBase(qsg_batch_s* b) {
+    const int n = (int)b->probs.size();
+    std::vector<uint64_t> keyBase(n + 1, 0);
+    for (int i = 0; i < n; i++)
+        keyBase[i + 1] = keyBase[i] + (b->probs[i] ? qs::sortKeys(b->probs[i]->tables.nCells) : 0);
+    b->sortKeys = keyBase[n];
+    if (b->sortKeys > b->sortKeysCap || !b->dSortCounts) {
+        for (void* d : {(void*)b->dSortCounts, (void*)b->dSortBlockSums})
+            if (d) (void)hipFree(d);
+        b->sortKeysCap = b->sortKeys + b->sortKeys / 2 + 1024;
+        b->dSortCounts = devAlloc<uint32_t>(b->sortKeysCap);
+        b->dSortBlockSums = devAlloc<uint32_t>(qs::scanBlocks(b->sortKeysCap));
+        b->graphsValid = false;
+    }
+    hipCheck(hipMemcpyAsync(b->dKeyBase, keyBase.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                            b->stream),
+             "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+// Puts problem p (or nothing) in slot i: its descriptor, buffer views and a
+// fresh per-slot state continuing from the problem's own cycle.  The slot
+// array's address and size do not change, so captured graphs stay valid.
+void setMember(qsg_batch_s* b, int i, qsg_problem_s* p) {
+    if (i < 0 || i >= (int)b->probs.size()) throw QsgError(QSG_ERR_ARG, "bad batch slot");
+    qs::BatchSlot slot;
+    qs::BatchState st;
+    memset(&slot, 0, sizeof(slot));
+    memset(&st, 0, sizeof(st));
+    slot.st = b->dState + i;
+    slot.dc = b->dCounters + i;
+    if (p) {
+        checkMember(b, p);
+        // members track through the batch's stacks; a problem run alone
+        // again reallocates its own on its next tracking pass
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        p->stacks.release();
+        slot.prob = p->dp;
+        slot.proc[0] = p->proc[0].v;
+        slot.proc[1] = p->proc[1].v;
+        slot.pop = p->pop.v;
+        slot.extra[0] = p->extra[0].v;
+        slot.extra[1] = p->extra[1].v;
+        slot.cap = p->pop.cap;
+        slot.loadBalance = (uint32_t)p->tables.loadBalance;
+        slot.sourcePerCycle = p->tables.sourcePerCycle;
+        slot.nParticles = p->tables.nParticles;
+        st.nCensus = (uint32_t)p->startCount;
+        st.cycle = (uint32_t)p->cycle;
+        st.active = 1;
+    }
+    b->probs[i] = p;
+    b->active[i] = p ? 1u : 0u;
+    b->expectedCycle[i] = p ? p->cycle : 0;
+    hipCheck(hipMemcpyAsync(b->dSlots + i, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(b->dState + i, &st, sizeof(st), hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    uploadKeyBase(b);
+}
+
+// A batch of `capacity` slots holding problems[0..n) (n <= capacity; the
+// other slots start empty).
+qsg_batch_s* createBatch(qsg_problem* problems, int n, int capacity, int device, qsgi::StackPool* pool) {
+    if (n < 0 || (n > 0 && !problems) || capacity < n || capacity <= 0)
+        throw QsgError(QSG_ERR_ARG, "a batch needs at least one slot and room for its problems");
     qsg_batch_s* b = new qsg_batch_s();
     if (pool) b->stacks = pool;
     try {
-        b->device = problems[0]->device;
+        b->device = n > 0 ? problems[0]->device : device;
         for (int i = 0; i < n; i++) {
-            qsg_problem_s* p = problems[i];
-            if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem in batch");
-            if (p->device != b->device) throw QsgError(QSG_ERR_ARG, "batch problems must share one device");
-            if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "batch problems must be at a cycle boundary");
-            if (p->cur != (p->cycle & 1))
-                throw QsgError(QSG_ERR_STATE, "problem buffer parity does not match its cycle");
-            b->probs.push_back(p);
-        }
-        // members track through the batch's stacks; a problem run alone
-        // again reallocates its own on its next tracking pass
-        for (qsg_problem_s* p : b->probs) {
-            hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-            p->stacks.release();
+            if (!problems[i]) throw QsgError(QSG_ERR_ARG, "NULL problem in batch");
+            checkMember(b, problems[i]);
         }
         hipCheck(hipSetDevice(b->device), "hipSetDevice");
         hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -112,56 +175,25 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
         hipCheck(hipEventCreate(&b->ev0), "hipEventCreate");
         hipCheck(hipEventCreate(&b->ev1), "hipEventCreate");
         hipCheck((hipError_t)qs::prepareDevice(qs::LaunchCtx{(void*)b->stream, b->numCUs}), "device set-up");
-        b->tune = problems[0]->tune;
-        b->dSlots = devAlloc<qs::BatchSlot>(n);
-        b->dState = devAlloc<qs::BatchState>(n);
-        b->dCounters = devAlloc<qs::DevCounters>(n);
-        b->dSrcPrefix = devAlloc<uint64_t>(n + 1);
-        b->dPopPrefix = devAlloc<uint64_t>(n + 1);
-        b->dPartials = devAlloc<double>((size_t)n * b->partsPerProblem);
-        {
-            std::vector<uint64_t> keyBase(n + 1, 0);
-            for (int i = 0; i < n; i++) keyBase[i + 1] = keyBase[i] + qs::sortKeys(b->probs[i]->tables.nCells);
-            b->sortKeys = keyBase[n];
-            b->dKeyBase = devAlloc<uint64_t>(n + 1);
-            hipCheck(hipMemcpy(b->dKeyBase, keyBase.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice),
-                     "hipMemcpy");
-            b->dSortPrefix = devAlloc<uint64_t>(n + 1);
-            b->dSortCounts = devAlloc<uint32_t>(b->sortKeys);
-            b->dSortBlockSums = devAlloc<uint32_t>(qs::scanBlocks(b->sortKeys));
-        }
-        b->dFluxSums = devAlloc<double>(n);
-        b->hState = hostAlloc<qs::BatchState>(n);
-        b->hCounters = hostAlloc<qs::DevCounters>(n);
-        b->hFluxSums = hostAlloc<double>(n);
-        std::vector<qs::BatchSlot> slots(n);
-        std::vector<qs::BatchState> st(n);
-        for (int i = 0; i < n; i++) {
-            qsg_problem_s* p = b->probs[i];
-            qs::BatchSlot& s = slots[i];
-            memset(&s, 0, sizeof(s));
-            s.prob = p->dp;
-            s.proc[0] = p->proc[0].v;
-            s.proc[1] = p->proc[1].v;
-            s.pop = p->pop.v;
-            s.extra[0] = p->extra[0].v;
-            s.extra[1] = p->extra[1].v;
-            s.cap = p->pop.cap;
-            s.loadBalance = (uint32_t)p->tables.loadBalance;
-            s.sourcePerCycle = p->tables.sourcePerCycle;
-            s.nParticles = p->tables.nParticles;
-            s.st = b->dState + i;
-            s.dc = b->dCounters + i;
-            memset(&st[i], 0, sizeof(st[i]));
-            st[i].nCensus = (uint32_t)p->startCount;
-            st[i].cycle = (uint32_t)p->cycle;
-            st[i].active = 1;
-        }
-        b->active.assign(n, 1);
-        for (int i = 0; i < n; i++) b->expectedCycle.push_back(b->probs[i]->cycle);
-        hipCheck(hipMemcpy(b->dSlots, slots.data(), n * sizeof(qs::BatchSlot), hipMemcpyHostToDevice), "hipMemcpy");
-        hipCheck(hipMemcpy(b->dState, st.data(), n * sizeof(qs::BatchState), hipMemcpyHostToDevice), "hipMemcpy");
-        hipCheck(hipMemset(b->dCounters, 0, n * sizeof(qs::DevCounters)), "hipMemset");
+        if (n > 0) b->tune = problems[0]->tune;
+        const int c = capacity;
+        b->probs.assign(c, nullptr);
+        b->active.assign(c, 0);
+        b->expectedCycle.assign(c, 0);
+        b->dSlots = devAlloc<qs::BatchSlot>(c);
+        b->dState = devAlloc<qs::BatchState>(c);
+        b->dCounters = devAlloc<qs::DevCounters>(c);
+        b->dSrcPrefix = devAlloc<uint64_t>(c + 1);
+        b->dPopPrefix = devAlloc<uint64_t>(c + 1);
+        b->dPartials = devAlloc<double>((size_t)c * b->partsPerProblem);
+        b->dKeyBase = devAlloc<uint64_t>(c + 1);
+        b->dSortPrefix = devAlloc<uint64_t>(c + 1);
+        b->dFluxSums = devAlloc<double>(c);
+        b->hState = hostAlloc<qs::BatchState>(c);
+        b->hCounters = hostAlloc<qs::DevCounters>(c);
+        b->hFluxSums = hostAlloc<double>(c);
+        hipCheck(hipMemset(b->dCounters, 0, c * sizeof(qs::DevCounters)), "hipMemset");
+        for (int i = 0; i < c; i++) setMember(b, i, i < n ? problems[i] : nullptr);
         hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
         destroyBatch(b);
@@ -226,7 +258,7 @@ void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& w
 // flux sums, tallies and states to the host.
 void enqueueFinal(qsg_batch_s* b, const qs::LaunchCtx& c) {
     const int n = (int)b->probs.size();
-    for (qsg_problem_s* p : b->probs) qs::launchFluence(c, p->dp);
+    qs::launchBatchFluence(c, b->dSlots, n);
     qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
     hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
              "hipMemcpyAsync");
@@ -265,8 +297,8 @@ bool sameShape(const qs::TrackTuning& a, const qs::TrackTuning& t) {
 // The step's two graphs, captured again whenever the launch shape or the
 // stack memory changed since they were captured.
 void ensureGraphs(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws) {
-    if (b->gInit && b->gFinal && sameShape(b->graphTune, b->tune) && b->graphStacks.base == ws.base &&
-        b->graphStacks.cap == ws.cap && b->graphStacks.nWaves == ws.nWaves)
+    if (b->gInit && b->gFinal && b->graphsValid && sameShape(b->graphTune, b->tune) &&
+        b->graphStacks.base == ws.base && b->graphStacks.cap == ws.cap && b->graphStacks.nWaves == ws.nWaves)
         return;
     for (hipGraphExec_t* g : {&b->gInit, &b->gFinal})
         if (*g) {
@@ -277,6 +309,7 @@ void ensureGraphs(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& 
     b->gFinal = capture(b, [&] { enqueueFinal(b, c); });
     b->graphTune = b->tune;
     b->graphStacks = ws;
+    b->graphsValid = true;
 }
 
 // One cycle of every active problem (cycleInit + cycleTracking +
@@ -288,7 +321,7 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
     hipCheck(hipSetDevice(b->device), "hipSetDevice");
     const int n = (int)b->probs.size();
     for (int i = 0; i < n; i++)
-        if (b->probs[i]->phase != 0 || b->probs[i]->cycle != b->expectedCycle[i])
+        if (b->probs[i] && (b->probs[i]->phase != 0 || b->probs[i]->cycle != b->expectedCycle[i]))
             throw QsgError(QSG_ERR_STATE, "a batch problem was advanced outside the batch");
     qs::LaunchCtx c{(void*)b->stream, b->numCUs};
     const long long blocks = qs::batchTrackBlocks(c, b->tune);
@@ -345,7 +378,7 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
     for (int i = 0; i < n; i++) {
         const qs::BatchState& st = b->hState[i];
         qsg_problem_s* p = b->probs[i];
-        if (!st.active) {
+        if (!p || !st.active) {
             if (balances) memset(&balances[i], 0, sizeof(qsg_balance));
             if (fluxSums) fluxSums[i] = 0.0;
             continue;
@@ -395,9 +428,14 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
 
 }  // namespace
 
-qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, qsgi::StackPool* pool) {
-    return createBatch(problems, n, pool);
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, int capacity, int device, qsgi::StackPool* pool) {
+    return createBatch(problems, n, capacity, device, pool);
 }
+void qsgiBatchSetMember(qsg_batch_s* b, int slot, qsg_problem_s* p) {
+    hipCheck(hipSetDevice(b->device), "hipSetDevice");
+    setMember(b, slot, p);
+}
+int qsgiBatchCapacity(const qsg_batch_s* b) { return (int)b->probs.size(); }
 void qsgiDestroyBatch(qsg_batch_s* b) { destroyBatch(b); }
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) { batchStep(b, balances, fluxSums); }
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t) { b->tune = t; }
@@ -409,7 +447,8 @@ int qsg_batch_create(qsg_problem* problems, int n, qsg_batch* out) {
     QSG_GUARD({
         if (!out) throw QsgError(QSG_ERR_ARG, "out is NULL");
         *out = nullptr;
-        *out = createBatch(problems, n, nullptr);
+        if (n <= 0 || !problems) throw QsgError(QSG_ERR_ARG, "a batch needs at least one problem");
+        *out = createBatch(problems, n, n, problems[0] ? problems[0]->device : 0, nullptr);
     })
 }
 
@@ -427,6 +466,7 @@ int qsg_batch_step(qsg_batch b, qsg_balance* balances, double* flux_sums) {
 int qsg_batch_set_active(qsg_batch b, int index, int active) {
     QSG_GUARD({
         if (!b || index < 0 || index >= (int)b->probs.size()) throw QsgError(QSG_ERR_ARG, "bad batch index");
+        if (active && !b->probs[index]) throw QsgError(QSG_ERR_STATE, "batch slot holds no problem");
         hipCheck(hipSetDevice(b->device), "hipSetDevice");
         uint32_t a = active ? 1u : 0u;
         hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");

@@ -140,8 +140,13 @@ void destroy(qsg_problem_s* p);
 }  // namespace qsgi
 
 // Batch internals used by the serving loop.
-// pool: stacks shared with the caller (NULL: the batch owns its stacks)
-qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, qsgi::StackPool* pool);
+// A batch with `capacity` slots, the first n holding problems[0..n) (the
+// rest empty); pool: stacks shared with the caller (NULL: the batch owns its
+// stacks).  Members are replaced in place with qsgiBatchSetMember (NULL
+// empties a slot) at a cycle boundary; captured graphs stay valid.
+qsg_batch_s* qsgiCreateBatch(qsg_problem* problems, int n, int capacity, int device, qsgi::StackPool* pool);
+void qsgiBatchSetMember(qsg_batch_s* b, int slot, qsg_problem_s* p);
+int qsgiBatchCapacity(const qsg_batch_s* b);
 void qsgiDestroyBatch(qsg_batch_s* b);
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums);
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t);

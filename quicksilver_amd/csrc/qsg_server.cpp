@@ -2,10 +2,12 @@
 // queued, admitted in arrival order while their device footprint fits the
 // HBM budget, kept resident on the device across cycles, advanced together
 // one cycle per qsg_server_step, and retired (device memory released) when
-// they have run their nSteps cycles.  Continuous batching: the batch is
-// rebuilt at a cycle boundary whenever a request joins or leaves, so work
-// arriving while others run shares their launches.  No reference
-// counterpart: the reference runs one problem per process.
+// they have run their nSteps cycles.  Continuous batching: a request joins
+// or leaves the running batch at a cycle boundary by taking or freeing one
+// of its slots, so work arriving while others run shares their launches and
+// the batch's captured graphs stay valid; the batch is rebuilt only when it
+// runs out of slots.  No reference counterpart: the reference runs one
+// problem per process.
 #include <cstring>
 #include <deque>
 #include <map>
@@ -30,6 +32,7 @@ struct Request {
     std::vector<qsg_balance> bal;
     std::vector<double> flux;
     std::string error;
+    int slot = -1;  // batch slot while running
 };
 
 }  // namespace
@@ -43,8 +46,7 @@ struct qsg_server_s {
     std::deque<Request*> queue;
     std::vector<Request*> active;
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    qsg_batch_s* batch = nullptr;
-    bool dirty = true;  // membership changed since the batch was built
+    qsg_batch_s* batch = nullptr;  // slots for the running requests, kept across steps
     qs::TrackTuning tune = qs::kDefaultTuning;
     bool haveTune = false;
     qsgi::StackPool stacks;  // shared by the successive batches
@@ -59,13 +61,42 @@ void dropBatch(qsg_server_s* s) {
 }
 
 void retire(qsg_server_s* s, Request* r, int state) {
+    if (r->slot >= 0 && s->batch) {
+        try {
+            qsgiBatchSetMember(s->batch, r->slot, nullptr);
+        } catch (const std::exception&) {
+            dropBatch(s);  // a device error: the next admission builds a new batch
+        }
+    }
+    r->slot = -1;
     if (r->prob) {
         qsgi::destroy(r->prob);
         r->prob = nullptr;
         s->used -= r->bytes;
     }
     r->state = state;
-    s->dirty = true;
+}
+
+// A batch slot for a newly admitted request.  The batch keeps its slots
+// (and captured graphs) across steps; when all are taken it is rebuilt with
+// twice as many, the running requests keeping their state.
+int freeSlot(qsg_server_s* s) {
+    if (s->batch) {
+        std::vector<bool> used(qsgiBatchCapacity(s->batch), false);
+        for (Request* r : s->active)
+            if (r->slot >= 0) used[r->slot] = true;
+        for (size_t i = 0; i < used.size(); i++)
+            if (!used[i]) return (int)i;
+    }
+    const int old = s->batch ? qsgiBatchCapacity(s->batch) : 0;
+    const int cap = old ? 2 * old : (s->maxActive > 0 ? s->maxActive : 16);
+    std::vector<qsg_problem> hs;
+    for (Request* r : s->active) hs.push_back(r->prob);
+    dropBatch(s);
+    s->batch = qsgiCreateBatch(hs.data(), (int)hs.size(), cap, s->device, &s->stacks);
+    if (s->haveTune) qsgiBatchSetTuning(s->batch, s->tune);
+    for (size_t i = 0; i < s->active.size(); i++) s->active[i]->slot = (int)i;
+    return (int)s->active.size();
 }
 
 // Admit queued requests in arrival order while they fit the budget (and the
@@ -87,11 +118,18 @@ void admit(qsg_server_s* s) {
             r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device, false);
             r->tables = qs::ProblemTables();
             s->used += r->bytes;
+            const int slot = freeSlot(s);
+            qsgiBatchSetMember(s->batch, slot, r->prob);
+            r->slot = slot;
             r->state = 1;
             s->active.push_back(r);
             s->stats.admitted++;
-            s->dirty = true;
         } catch (const std::exception& e) {
+            if (r->prob) {
+                qsgi::destroy(r->prob);
+                r->prob = nullptr;
+                s->used -= r->bytes;
+            }
             r->error = e.what();
             r->state = 3;
             s->stats.rejected++;
@@ -102,17 +140,9 @@ void admit(qsg_server_s* s) {
 void serverStep(qsg_server_s* s) {
     admit(s);
     if (s->active.empty()) return;
-    if (s->dirty || !s->batch) {
-        dropBatch(s);
-        std::vector<qsg_problem> hs;
-        for (Request* r : s->active) hs.push_back(r->prob);
-        s->batch = qsgiCreateBatch(hs.data(), (int)hs.size(), &s->stacks);
-        if (s->haveTune) qsgiBatchSetTuning(s->batch, s->tune);
-        s->dirty = false;
-    }
-    const size_t n = s->active.size();
-    std::vector<qsg_balance> bal(n);
-    std::vector<double> flux(n);
+    const int cap = qsgiBatchCapacity(s->batch);
+    std::vector<qsg_balance> bal(cap);
+    std::vector<double> flux(cap);
     try {
         qsgiBatchStep(s->batch, bal.data(), flux.data());
     } catch (...) {
@@ -127,11 +157,10 @@ void serverStep(qsg_server_s* s) {
     }
     s->stats.steps++;
     std::vector<Request*> still;
-    for (size_t i = 0; i < n; i++) {
-        Request* r = s->active[i];
-        r->bal.push_back(bal[i]);
-        r->flux.push_back(flux[i]);
-        s->stats.segments += bal[i].num_segments;
+    for (Request* r : s->active) {
+        r->bal.push_back(bal[r->slot]);
+        r->flux.push_back(flux[r->slot]);
+        s->stats.segments += bal[r->slot].num_segments;
         if ((int)r->bal.size() >= r->nSteps) {
             retire(s, r, 2);
             s->stats.completed++;
@@ -140,7 +169,6 @@ void serverStep(qsg_server_s* s) {
         }
     }
     s->active.swap(still);
-    if (s->dirty) dropBatch(s);
 }
 
 Request* find(qsg_server_s* s, uint64_t id) {
@@ -169,8 +197,8 @@ int qsg_server_create(int device, uint64_t hbm_budget_bytes, int max_active, qsg
 int qsg_server_destroy(qsg_server s) {
     QSG_GUARD({
         if (!s) return QSG_OK;
-        dropBatch(s);
         for (Request* r : s->active) retire(s, r, 3);
+        dropBatch(s);
         (void)hipSetDevice(s->device);
         s->stacks.release();
         delete s;

@@ -64,3 +64,29 @@ def test_request_larger_than_budget_is_rejected():
         s.step()
         assert s.status(rid)[0] == q.Server.FAILED
         assert s.stats()["rejected"] == 1
+
+
+@pytest.mark.gpu
+def test_batch_slots_grow_and_are_reused():
+    """More simultaneous requests than the first batch has slots (16): the
+    batch grows while running requests keep their state; later arrivals
+    take slots freed by finished requests.  Every result still equals the
+    reference's."""
+    print("[server] slot growth and reuse", flush=True)
+    tiny, short = load_golden("tiny_cube"), load_golden("small_cube_short")
+    texts = {n: open(g["deck_path"]).read() for n, g in (("tiny_cube", tiny), ("small_cube_short", short))}
+    with q.Server(64 << 30) as s:
+        ids = [("tiny_cube", s.submit(texts["tiny_cube"])) for _ in range(20)]
+        s.step()
+        ids += [("small_cube_short", s.submit(texts["small_cube_short"])) for _ in range(3)]
+        for _ in range(3):
+            s.step()
+        ids += [("tiny_cube", s.submit(texts["tiny_cube"])) for _ in range(5)]
+        for _ in range(50):
+            active, queued = s.step()
+            if active == 0 and queued == 0:
+                break
+        for name, rid in ids:
+            assert s.status(rid)[0] == q.Server.DONE, name
+            _check(name, tiny if name == "tiny_cube" else short, s.result(rid))
+        assert s.stats()["completed"] == len(ids)

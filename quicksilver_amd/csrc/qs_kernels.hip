@@ -155,6 +155,38 @@ constexpr FacetTable makeFacetTable() {
 
 constexpr FacetTable kFacet = makeFacetTable();
 
+// Which triangle of a face lies on which edge of the face.  Face (AX,
+// side) holds facets 8*AX + 4*side + T, T = 0..3; each is two corners plus
+// the face centre, and the two corners share one edge of the face.  In the
+// face's projected (u, v) axes (AX 0 -> (y,z), 1 -> (z,x), 2 -> (x,y)),
+// edge 0 is u = U0, 1 u = U1, 2 v = V0, 3 v = V1.  kEdgeTri[AX][side][edge]
+// = T.
+struct EdgeTable {
+    int tri[3][2][4];
+};
+
+constexpr EdgeTable makeEdgeTable() {
+    EdgeTable e{};
+    for (int ax = 0; ax < 3; ax++) {
+        const int u = ax == 0 ? 1 : (ax == 1 ? 2 : 0), v = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+        for (int side = 0; side < 2; side++)
+            for (int t = 0; t < 4; t++) {
+                const int f = 8 * ax + 4 * side + t;
+                const int u0 = kFacet.code[f][0][u], u1 = kFacet.code[f][1][u];
+                const int v0 = kFacet.code[f][0][v], v1 = kFacet.code[f][1][v];
+                int edge = -1;
+                if (u0 == 0 && u1 == 0) edge = 0;
+                if (u0 == 1 && u1 == 1) edge = 1;
+                if (v0 == 0 && v1 == 0) edge = 2;
+                if (v0 == 1 && v1 == 1) edge = 3;
+                e.tri[ax][side][edge] = t;
+            }
+    }
+    return e;
+}
+
+constexpr EdgeTable kEdgeTri = makeEdgeTable();
+
 // Value of one node coordinate for a compile-time code.
 template <int AXIS, int CODE>
 __device__ __forceinline__ double node_value(const Cell9& c) {
@@ -361,6 +393,138 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
     }
 }
 
+// The four triangles of an exit face visited with one probe.  All four lie
+// in the face plane (their stored planes differ only in the last bits), so
+// the exact intersection point of any of them is within ~1e-15 of the
+// probe's, the intersection with triangle 0's plane.  Only a triangle whose
+// region, widened by `margin` (far above the reference's 1e-9 tolerances),
+// holds the probe point can pass the reference's containment test; every
+// other one returns kHuge.  The candidates (one, or two near a diagonal) get
+// the reference's exact test with their own plane, and every result is
+// merged in facet order, so the outcome is the same as testing all four.
+// Lanes for which the probe cannot decide -- particle on or behind the
+// plane, a near-grazing direction, a point near the face centre -- test all
+// four triangles instead.
+template <int AX>
+__device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* pnd, const Cell9& c9,
+                                                double planeTol, double x, double y, double z, double a,
+                                                double b, double g) {
+    constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
+    constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
+    const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
+    const bool plus = dAx > 0.0, minus = dAx < 0.0;
+    if (!(plus || minus)) {
+        // direction parallel to both faces of this axis: dot = 0, all kHuge
+#pragma unroll
+        for (int f = 8 * AX; f < 8 * AX + 8; f++) fn.add(kHuge, f);
+        return;
+    }
+    const int base = 8 * AX + (plus ? 0 : 4);
+    const double xa = AX == 0 ? x : (AX == 1 ? y : z);
+    const double N0 = pnd[2 * base], D0 = pnd[2 * base + 1];
+    const double dot0 = N0 * dAx;
+    const double num0 = -1.0 * (N0 * xa + D0);
+    const double U0 = AX == 0 ? c9.Y0 : (AX == 1 ? c9.Z0 : c9.X0);
+    const double U1 = AX == 0 ? c9.Y1 : (AX == 1 ? c9.Z1 : c9.X1);
+    const double Uh = AX == 0 ? c9.Yh : (AX == 1 ? c9.Zh : c9.Xh);
+    const double V0 = AX == 0 ? c9.Z0 : (AX == 1 ? c9.X0 : c9.Y0);
+    const double V1 = AX == 0 ? c9.Z1 : (AX == 1 ? c9.X1 : c9.Y1);
+    const double Vh = AX == 0 ? c9.Zh : (AX == 1 ? c9.Xh : c9.Yh);
+    const double hu = 0.5 * (U1 - U0), hv = 0.5 * (V1 - V0);
+    const double margin = fmax(1e-6 * fmin(hu, hv), 1e-7);
+    // bound on how far the four numerators can differ (last-bit differences
+    // of the planes, times the coordinates involved)
+    const double errN = 4e-16 * (fabs(xa) + fabs(D0)) + 1e-300;
+    bool unsure = !(num0 > 1e3 * errN) || !(errN < 1e-3 * margin * dot0);
+    double du = 0.0, dv = 0.0;
+    if (!unsure) {
+        const double d0 = num0 / dot0;
+        const double iu = (U == 0 ? x : (U == 1 ? y : z)) + d0 * (U == 0 ? a : (U == 1 ? b : g));
+        const double iv = (V == 0 ? x : (V == 1 ? y : z)) + d0 * (V == 0 ? a : (V == 1 ? b : g));
+        du = iu - Uh;
+        dv = iv - Vh;
+    }
+    // The face's diagonals split it into the four triangles (the one on the
+    // u = U1 edge is the wedge s1 > 0, s2 > 0 below, and so on).  A triangle
+    // across a diagonal lies beyond that diagonal's line, so it is a
+    // candidate only if the point is within `margin` of the line
+    // (|s| <= margin * (hu + hv) >= margin * |diagonal direction|).  Within
+    // margin of both diagonals -- the centre, where all four meet -- the
+    // probe decides nothing.
+    const double s1 = du * hv - dv * hu, s2 = du * hv + dv * hu;
+    const double thr = margin * (hu + hv);
+    const bool near1 = fabs(s1) <= thr, near2 = fabs(s2) <= thr;
+    if (near1 && near2) unsure = true;
+    if (unsure) {
+        // the reference's test on all four triangles (and kHuge for the
+        // opposite face), in facet order
+        visit_facet<AX, 0>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
+        visit_facet<AX, 1>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
+        visit_facet<AX, 2>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
+        visit_facet<AX, 3>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
+        return;
+    }
+    // candidates: none outside the face (widened by margin); else the
+    // point's wedge, and the wedge across a diagonal it is near
+    int c1 = -1, c2 = -1;
+    if (fabs(du) <= hu + margin && fabs(dv) <= hv + margin) {
+        const int side = plus ? 0 : 1;
+        // edge numbering of the edge table: 0 u = U0, 1 u = U1, 2 v = V0, 3 v = V1
+        const int wedge = s1 > 0.0 ? (s2 > 0.0 ? 1 : 2) : (s2 > 0.0 ? 3 : 0);
+        int other = -1;
+        if (near1) other = s2 > 0.0 ? (wedge == 1 ? 3 : 1) : (wedge == 2 ? 0 : 2);
+        else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
+        const int t1 = kEdgeTri.tri[AX][side][wedge];
+        c1 = t1;
+        if (other >= 0) {
+            const int t2 = kEdgeTri.tri[AX][side][other];
+            c1 = t1 < t2 ? t1 : t2;
+            c2 = t1 < t2 ? t2 : t1;
+        }
+    }
+    double d1 = kHuge, d2 = kHuge;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+        const int t = pass == 0 ? c1 : c2;
+        if (t < 0) continue;
+        const int f = base + t;
+        const double N = pnd[2 * f], D = pnd[2 * f + 1];
+        const double dot = N * dAx;
+        double d = kHuge;
+        if (dot > 0.0) {
+            const uint32_t code = c_facetCode[f];
+            const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
+            const double p0u = U == 0 ? n0.x : (U == 1 ? n0.y : n0.z), p0v = V == 0 ? n0.x : (V == 1 ? n0.y : n0.z);
+            const double p1u = U == 0 ? n1.x : (U == 1 ? n1.y : n1.z), p1v = V == 0 ? n1.x : (V == 1 ? n1.y : n1.z);
+            const double p2u = U == 0 ? n2.x : (U == 1 ? n2.y : n2.z), p2v = V == 0 ? n2.x : (V == 1 ? n2.y : n2.z);
+            d = distance_to_face_facet<AX>(planeTol, dot, N, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
+        }
+        if (pass == 0) d1 = d;
+        else d2 = d;
+    }
+    // merge in facet order: the exit face's four, then the opposite face's
+    // four are all kHuge (dot < 0); only the order of the exit face's entries
+    // matters for ties, and the opposite face only raises the kHuge index
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double d = t == c1 ? d1 : (t == c2 ? d2 : kHuge);
+        if (plus) {
+            fn.add(d, 8 * AX + t);
+        } else {
+            fn.add(kHuge, 8 * AX + t);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double d = t == c1 ? d1 : (t == c2 ? d2 : kHuge);
+        if (minus) {
+            fn.add(d, 8 * AX + 4 + t);
+        } else {
+            fn.add(kHuge, 8 * AX + 4 + t);
+        }
+    }
+}
+
 // MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G (MCT.cc:87-137, 550-621) with
 // the retry of MCT_Nearest_Facet_Find_Nearest (MCT.cc:487-536): when no
 // facet bounds the particle it is nudged toward the cell centre and the
@@ -376,9 +540,9 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
         FindNearest fn = {1e80, -1, -1, -kHuge, 0};
-        visit_axis<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-        visit_axis<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-        visit_axis<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        visit_exit_face<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        visit_exit_face<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        visit_exit_face<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
         nf = fn.result();
     }
     if (nf.d == kHuge) {

@@ -436,13 +436,13 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
     // of the planes, times the coordinates involved)
     const double errN = 4e-16 * (fabs(xa) + fabs(D0)) + 1e-300;
     bool unsure = !(num0 > 1e3 * errN) || !(errN < 1e-3 * margin * dot0);
+    // probe point relative to the face centre, scaled by dot0 > 0 so that no
+    // division is needed: du = (u - Uh) + (num0 / dot0) * dir_u, times dot0
+    // (rounding here is ~1e-16 of quantities the margin exceeds by 1e6)
     double du = 0.0, dv = 0.0;
     if (!unsure) {
-        const double d0 = num0 / dot0;
-        const double iu = (U == 0 ? x : (U == 1 ? y : z)) + d0 * (U == 0 ? a : (U == 1 ? b : g));
-        const double iv = (V == 0 ? x : (V == 1 ? y : z)) + d0 * (V == 0 ? a : (V == 1 ? b : g));
-        du = iu - Uh;
-        dv = iv - Vh;
+        du = ((U == 0 ? x : (U == 1 ? y : z)) - Uh) * dot0 + num0 * (U == 0 ? a : (U == 1 ? b : g));
+        dv = ((V == 0 ? x : (V == 1 ? y : z)) - Vh) * dot0 + num0 * (V == 0 ? a : (V == 1 ? b : g));
     }
     // The face's diagonals split it into the four triangles (the one on the
     // u = U1 edge is the wedge s1 > 0, s2 > 0 below, and so on).  A triangle
@@ -450,9 +450,9 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
     // candidate only if the point is within `margin` of the line
     // (|s| <= margin * (hu + hv) >= margin * |diagonal direction|).  Within
     // margin of both diagonals -- the centre, where all four meet -- the
-    // probe decides nothing.
+    // probe decides nothing.  Lengths below are scaled by dot0 like du, dv.
     const double s1 = du * hv - dv * hu, s2 = du * hv + dv * hu;
-    const double thr = margin * (hu + hv);
+    const double thr = margin * (hu + hv) * dot0;
     const bool near1 = fabs(s1) <= thr, near2 = fabs(s2) <= thr;
     if (near1 && near2) unsure = true;
     if (unsure) {
@@ -467,7 +467,7 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
     // candidates: none outside the face (widened by margin); else the
     // point's wedge, and the wedge across a diagonal it is near
     int c1 = -1, c2 = -1;
-    if (fabs(du) <= hu + margin && fabs(dv) <= hv + margin) {
+    if (fabs(du) <= (hu + margin) * dot0 && fabs(dv) <= (hv + margin) * dot0) {
         const int side = plus ? 0 : 1;
         // edge numbering of the edge table: 0 u = U0, 1 u = U1, 2 v = V0, 3 v = V1
         const int wedge = s1 > 0.0 ? (s2 > 0.0 ? 1 : 2) : (s2 > 0.0 ? 3 : 0);

@@ -393,32 +393,30 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
     }
 }
 
-// The four triangles of an exit face visited with one probe.  All four lie
-// in the face plane (their stored planes differ only in the last bits), so
-// the exact intersection point of any of them is within ~1e-15 of the
-// probe's, the intersection with triangle 0's plane.  Only a triangle whose
-// region, widened by `margin` (far above the reference's 1e-9 tolerances),
-// holds the probe point can pass the reference's containment test; every
-// other one returns kHuge.  The candidates (one, or two near a diagonal) get
-// the reference's exact test with their own plane, and every result is
-// merged in facet order, so the outcome is the same as testing all four.
-// Lanes for which the probe cannot decide -- particle on or behind the
-// plane, a near-grazing direction, a point near the face centre -- test all
-// four triangles instead.
+// Probe of one exit face.  The four triangles of a face lie in the face
+// plane (their stored planes differ only in the last bits), so the exact
+// intersection point with any of them is within ~1e-15 of the probe's, the
+// intersection with triangle 0's plane.  Only a triangle whose region,
+// widened by `margin` (far above the reference's 1e-9 tolerances), holds the
+// probe point can pass the reference's containment test; every other one
+// returns kHuge.  The probe names those candidates (one, or two near a
+// diagonal; none when the point is outside the face) or, when it cannot
+// decide -- particle on or behind the plane, a near-grazing direction, a
+// point near the face centre -- says so.
+struct FaceProbe {
+    int c1, c2;  // candidate facet indices, c1 < c2, or -1
+    bool unsure;
+};
+
 template <int AX>
-__device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* pnd, const Cell9& c9,
-                                                double planeTol, double x, double y, double z, double a,
-                                                double b, double g) {
+__device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Cell9& c9, double x, double y,
+                                                     double z, double a, double b, double g) {
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
+    FaceProbe r = {-1, -1, false};
     const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
-    if (!(plus || minus)) {
-        // direction parallel to both faces of this axis: dot = 0, all kHuge
-#pragma unroll
-        for (int f = 8 * AX; f < 8 * AX + 8; f++) fn.add(kHuge, f);
-        return;
-    }
+    if (!(plus || minus)) return r;  // parallel to both faces: dot = 0, all kHuge
     const int base = 8 * AX + (plus ? 0 : 4);
     const double xa = AX == 0 ? x : (AX == 1 ? y : z);
     const double N0 = pnd[2 * base], D0 = pnd[2 * base + 1];
@@ -435,15 +433,15 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
     // bound on how far the four numerators can differ (last-bit differences
     // of the planes, times the coordinates involved)
     const double errN = 4e-16 * (fabs(xa) + fabs(D0)) + 1e-300;
-    bool unsure = !(num0 > 1e3 * errN) || !(errN < 1e-3 * margin * dot0);
+    if (!(num0 > 1e3 * errN) || !(errN < 1e-3 * margin * dot0)) {
+        r.unsure = true;
+        return r;
+    }
     // probe point relative to the face centre, scaled by dot0 > 0 so that no
     // division is needed: du = (u - Uh) + (num0 / dot0) * dir_u, times dot0
     // (rounding here is ~1e-16 of quantities the margin exceeds by 1e6)
-    double du = 0.0, dv = 0.0;
-    if (!unsure) {
-        du = ((U == 0 ? x : (U == 1 ? y : z)) - Uh) * dot0 + num0 * (U == 0 ? a : (U == 1 ? b : g));
-        dv = ((V == 0 ? x : (V == 1 ? y : z)) - Vh) * dot0 + num0 * (V == 0 ? a : (V == 1 ? b : g));
-    }
+    const double du = ((U == 0 ? x : (U == 1 ? y : z)) - Uh) * dot0 + num0 * (U == 0 ? a : (U == 1 ? b : g));
+    const double dv = ((V == 0 ? x : (V == 1 ? y : z)) - Vh) * dot0 + num0 * (V == 0 ? a : (V == 1 ? b : g));
     // The face's diagonals split it into the four triangles (the one on the
     // u = U1 edge is the wedge s1 > 0, s2 > 0 below, and so on).  A triangle
     // across a diagonal lies beyond that diagonal's line, so it is a
@@ -454,19 +452,10 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
     const double s1 = du * hv - dv * hu, s2 = du * hv + dv * hu;
     const double thr = margin * (hu + hv) * dot0;
     const bool near1 = fabs(s1) <= thr, near2 = fabs(s2) <= thr;
-    if (near1 && near2) unsure = true;
-    if (unsure) {
-        // the reference's test on all four triangles (and kHuge for the
-        // opposite face), in facet order
-        visit_facet<AX, 0>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
-        visit_facet<AX, 1>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
-        visit_facet<AX, 2>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
-        visit_facet<AX, 3>(fn, pnd, c9, planeTol, x, y, z, a, b, g, plus, minus);
-        return;
+    if (near1 && near2) {
+        r.unsure = true;
+        return r;
     }
-    // candidates: none outside the face (widened by margin); else the
-    // point's wedge, and the wedge across a diagonal it is near
-    int c1 = -1, c2 = -1;
     if (fabs(du) <= (hu + margin) * dot0 && fabs(dv) <= (hv + margin) * dot0) {
         const int side = plus ? 0 : 1;
         // edge numbering of the edge table: 0 u = U0, 1 u = U1, 2 v = V0, 3 v = V1
@@ -475,54 +464,56 @@ __device__ __forceinline__ void visit_exit_face(FindNearest& fn, const double* p
         if (near1) other = s2 > 0.0 ? (wedge == 1 ? 3 : 1) : (wedge == 2 ? 0 : 2);
         else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
         const int t1 = kEdgeTri.tri[AX][side][wedge];
-        c1 = t1;
+        r.c1 = base + t1;
         if (other >= 0) {
             const int t2 = kEdgeTri.tri[AX][side][other];
-            c1 = t1 < t2 ? t1 : t2;
-            c2 = t1 < t2 ? t2 : t1;
+            r.c1 = base + (t1 < t2 ? t1 : t2);
+            r.c2 = base + (t1 < t2 ? t2 : t1);
         }
     }
-    double d1 = kHuge, d2 = kHuge;
-#pragma unroll 1
-    for (int pass = 0; pass < 2; pass++) {
-        const int t = pass == 0 ? c1 : c2;
-        if (t < 0) continue;
-        const int f = base + t;
-        const double N = pnd[2 * f], D = pnd[2 * f + 1];
-        const double dot = N * dAx;
-        double d = kHuge;
-        if (dot > 0.0) {
-            const uint32_t code = c_facetCode[f];
-            const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
-            const double p0u = U == 0 ? n0.x : (U == 1 ? n0.y : n0.z), p0v = V == 0 ? n0.x : (V == 1 ? n0.y : n0.z);
-            const double p1u = U == 0 ? n1.x : (U == 1 ? n1.y : n1.z), p1v = V == 0 ? n1.x : (V == 1 ? n1.y : n1.z);
-            const double p2u = U == 0 ? n2.x : (U == 1 ? n2.y : n2.z), p2v = V == 0 ? n2.x : (V == 1 ? n2.y : n2.z);
-            d = distance_to_face_facet<AX>(planeTol, dot, N, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
-        }
-        if (pass == 0) d1 = d;
-        else d2 = d;
-    }
-    // merge in facet order: the exit face's four, then the opposite face's
-    // four are all kHuge (dot < 0); only the order of the exit face's entries
-    // matters for ties, and the opposite face only raises the kHuge index
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const double d = t == c1 ? d1 : (t == c2 ? d2 : kHuge);
-        if (plus) {
-            fn.add(d, 8 * AX + t);
-        } else {
-            fn.add(kHuge, 8 * AX + t);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const double d = t == c1 ? d1 : (t == c2 ? d2 : kHuge);
-        if (minus) {
-            fn.add(d, 8 * AX + 4 + t);
-        } else {
-            fn.add(kHuge, 8 * AX + 4 + t);
-        }
-    }
+    return r;
+}
+
+// The reference's exact test (distance_to_face_facet) for a facet chosen
+// per lane at run time: the projection axes are selected per lane, so the
+// whole wave runs one copy of the test whatever face each lane's candidate
+// lies on.
+__device__ __forceinline__ double exact_facet_test(int f, const double* pnd, const Cell9& c9, double planeTol,
+                                                   double x, double y, double z, double a, double b, double g) {
+    const int ax = f >> 3;
+    const double N = pnd[2 * f], D = pnd[2 * f + 1];
+    const double dAx = ax == 0 ? a : (ax == 1 ? b : g);
+    const double dot = N * dAx;
+    if (!(dot > 0.0)) return kHuge;
+    const double tol = 1e-9;
+    const double xa = ax == 0 ? x : (ax == 1 ? y : z);
+    double numerator = -1.0 * (N * xa + D);
+    if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
+    double distance = numerator / dot;
+    double ix = x + distance * a;
+    double iy = y + distance * b;
+    double iz = z + distance * g;
+    const double iu = ax == 0 ? iy : (ax == 1 ? iz : ix);
+    const double iv = ax == 0 ? iz : (ax == 1 ? ix : iy);
+    const uint32_t code = c_facetCode[f];
+    const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
+    const double p0u = ax == 0 ? n0.y : (ax == 1 ? n0.z : n0.x), p0v = ax == 0 ? n0.z : (ax == 1 ? n0.x : n0.y);
+    const double p1u = ax == 0 ? n1.y : (ax == 1 ? n1.z : n1.x), p1v = ax == 0 ? n1.z : (ax == 1 ? n1.x : n1.y);
+    const double p2u = ax == 0 ? n2.y : (ax == 1 ? n2.z : n2.x), p2v = ax == 0 ? n2.z : (ax == 1 ? n2.x : n2.y);
+    double iuP = iu + tol, iuM = iu - tol, ivP = iv + tol, ivM = iv - tol;
+    if (p0u > iuP && p1u > iuP && p2u > iuP) return kHuge;
+    if (p0u < iuM && p1u < iuM && p2u < iuM) return kHuge;
+    if (p0v > ivP && p1v > ivP && p2v > ivP) return kHuge;
+    if (p0v < ivM && p1v < ivM && p2v < ivM) return kHuge;
+    double cross1 = (p1u - p0u) * (iv - p0v) - (p1v - p0v) * (iu - p0u);
+    double cross2 = (p2u - p1u) * (iv - p1v) - (p2v - p1v) * (iu - p1u);
+    double cross0 = (p0u - p2u) * (iv - p2v) - (p0v - p2v) * (iu - p2u);
+    double s = cross0 + cross1 + cross2;
+    double ctol = 1e-9 * (s < 0 ? -s : s);
+    if ((cross0 > -ctol && cross1 > -ctol && cross2 > -ctol) ||
+        (cross0 < ctol && cross1 < ctol && cross2 < ctol))
+        return distance;
+    return kHuge;
 }
 
 // MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G (MCT.cc:87-137, 550-621) with
@@ -540,9 +531,47 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
         FindNearest fn = {1e80, -1, -1, -kHuge, 0};
-        visit_exit_face<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-        visit_exit_face<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-        visit_exit_face<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        const FaceProbe f0 = probe_exit_face<0>(pnd, c9, x, y, z, a, b, g);
+        const FaceProbe f1 = probe_exit_face<1>(pnd, c9, x, y, z, a, b, g);
+        const FaceProbe f2 = probe_exit_face<2>(pnd, c9, x, y, z, a, b, g);
+        // the lane's candidates in facet order (the faces are probed in
+        // facet order, each face's candidates ascending), at most three
+        int n = 0, q0 = -1, q1 = -1, q2 = -1;
+        const int cs[6] = {f0.c1, f0.c2, f1.c1, f1.c2, f2.c1, f2.c2};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            if (cs[k] >= 0) {
+                if (n == 0) q0 = cs[k];
+                else if (n == 1) q1 = cs[k];
+                else if (n == 2) q2 = cs[k];
+                n++;
+            }
+        }
+        if (f0.unsure || f1.unsure || f2.unsure || n > 3) {
+            // the reference's test on the 12 facets of the three exit faces
+            visit_axis<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+        } else {
+            // exact tests of the candidates, merged in facet order; every
+            // other facet is kHuge, which only sets the largest kHuge index
+            bool fin0 = false, fin1 = false, fin2 = false;
+#pragma unroll 1
+            for (int i = 0; i < n; i++) {
+                const int f = i == 0 ? q0 : (i == 1 ? q1 : q2);
+                const double d = exact_facet_test(f, pnd, c9, planeTol, x, y, z, a, b, g);
+                fn.add(d, f);
+                const bool fin = d != kHuge;
+                if (i == 0) fin0 = fin;
+                else if (i == 1) fin1 = fin;
+                else fin2 = fin;
+            }
+            int h = 23;
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                if ((fin0 && q0 == h) || (fin1 && q1 == h) || (fin2 && q2 == h)) h--;
+            fn.add(kHuge, h);
+        }
         nf = fn.result();
     }
     if (nf.d == kHuge) {

@@ -674,6 +674,28 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // Reserve `count` (0..4) consecutive slots per active lane with one atomic
 // per wave; returns this lane's first slot.
+// A collision output stored before its updateTrajectory: the pre-collision
+// position, direction (in the velocity slots), weight and time to census,
+// its own seed, and the sampled energy and scattering cosine (in the energy
+// and mean-free-path slots); the cell is stored complemented (negative) to
+// mark the record.  Loading it runs updateTrajectory and then the store and
+// load the reference performs (direction re-derived from the velocity), so
+// the particle is the same as if it had been updated before the store.
+__device__ __forceinline__ void store_deferred(const Particles& o, uint32_t i, const Lane& L, uint64_t seed,
+                                               double energy, double angle, int cell) {
+    o.x()[i] = L.x;
+    o.y()[i] = L.y;
+    o.z()[i] = L.z;
+    o.vx()[i] = L.a;
+    o.vy()[i] = L.b;
+    o.vz()[i] = L.g;
+    o.E()[i] = energy;
+    o.w()[i] = L.w;
+    o.tc()[i] = L.tc;
+    o.nmfp()[i] = angle;
+    o.seed()[i] = seed;
+    o.cell()[i] = ~cell;
+}
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t count) {
     uint64_t act = __ballot(1);
     uint64_t b0 = __ballot(count > 0), b1 = __ballot(count > 1), b2 = __ballot(count > 2),
@@ -946,6 +968,12 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     uint32_t qBase = 0, qEnd = 0;
     bool exhausted = false;
     bool active = false;
+    // Per-lane work carried to the update block at the top of the next
+    // iteration: updateTrajectory (with the energy and scattering cosine
+    // held in L.E and L.nmfp) for a scatter survivor or a loaded collision
+    // record, and the energy-group search of every loaded or updated
+    // particle.  One copy of each serves all lanes that need it.
+    bool upd = false, rec = false, grp = false;
     Lane L = {};
     uint32_t histSeg = 0;
     // wave-uniform (scalar) balance counters, updated from ballots at the
@@ -1000,20 +1028,31 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                     L.nmfp = src.nmfp()[idx];
                     L.seed = src.seed()[idx];
                     int cell = src.cell()[idx];
+                    rec = cell < 0;
+                    if (rec) cell = ~cell;
                     L.ci = cell % P.nx;
                     L.cj = (cell / P.nx) % P.ny;
                     L.ck = cell / (P.nx * P.ny);
-                    L.speed = vlength(L.vx, L.vy, L.vz);
-                    if (L.speed > 0) {
-                        double factor = 1.0 / L.speed;
-                        L.a = factor * L.vx;
-                        L.b = factor * L.vy;
-                        L.g = factor * L.vz;
+                    if (rec) {
+                        // a collision output: its direction is in the
+                        // velocity slots until updateTrajectory runs
+                        L.a = L.vx;
+                        L.b = L.vy;
+                        L.g = L.vz;
+                        upd = true;
                     } else {
-                        L.a = L.b = L.g = 0.0;
+                        L.speed = vlength(L.vx, L.vy, L.vz);
+                        if (L.speed > 0) {
+                            double factor = 1.0 / L.speed;
+                            L.a = factor * L.vx;
+                            L.b = factor * L.vy;
+                            L.g = factor * L.vz;
+                        } else {
+                            L.a = L.b = L.g = 0.0;
+                        }
+                        if (L.tc <= 0.0) L.tc += P.dt;
                     }
-                    if (L.tc <= 0.0) L.tc += P.dt;
-                    L.group = energy_group(T.energies, G, L.E);
+                    grp = true;
                     histSeg = 0;
                     active = true;
                 }
@@ -1026,6 +1065,30 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         // room for this iteration's pushes (uniform: decided before any lane diverges)
         const bool stackOk = top + kStackMargin <= scap;
         if (__ballot(active) == 0) break;  // drained
+        // ---- updateTrajectory (CollisionEvent.cc:135-140) of scatter
+        // survivors and loaded collision records ----
+        if (upd) {
+            update_trajectory(L, L.E, L.nmfp);
+            L.speed = vlength(L.vx, L.vy, L.vz);
+            if (rec) {
+                // the record's store and reload (MC_Load_Particle)
+                if (L.speed > 0) {
+                    double factor = 1.0 / L.speed;
+                    L.a = factor * L.vx;
+                    L.b = factor * L.vy;
+                    L.g = factor * L.vz;
+                } else {
+                    L.a = L.b = L.g = 0.0;
+                }
+                if (L.tc <= 0.0) L.tc += P.dt;
+            }
+            upd = false;
+            rec = false;
+        }
+        if (grp) {
+            L.group = energy_group(T.energies, G, L.E);
+            grp = false;
+        }
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
         if (active) {
@@ -1134,8 +1197,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             const int type = T.reactType[mat * P.maxR + selR];
             // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88).
             // The outgoing (energy, angle) pairs are drawn here only to
-            // advance the seed; the loop below replays them from s0, so
-            // they are not held in registers across the secondaries.
+            // advance the seed; each output replays its pair from s0 below.
             const uint64_t s0 = L.seed;
             int nOut = 0;
             if (type == kScatter) {
@@ -1165,22 +1227,21 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             if (nOut == 0) {
                 active = false;
             } else {
-                // outputs of a multi-particle collision go to the wave's
-                // stack when it has room, else to the pass's extra queue
+                // Outputs: secondaries k = 1..nOut-1 are copies of the
+                // pre-collision parent with spawned seeds and outgoing pair k
+                // (CollisionEvent.cc:125-133); the parent takes pair 0 and the
+                // seed left after the spawns (CollisionEvent.cc:135).  A
+                // single survivor stays in the lane; several go to the wave's
+                // stack (or the pass's extra queue when the stack is full) as
+                // records whose updateTrajectory runs when they are loaded.
                 const Particles dst = stackOk ? stack : io.extra;
                 const uint32_t dcap = stackOk ? scap : io.extraCap;
                 uint32_t slot = 0;
                 if (nOut > 1) slot = stackOk ? wave_push(sTopW, (uint32_t)nOut) : wave_append(io.nExtra, (uint32_t)nOut);
-                // secondaries k = 1..nOut-1 are copies of the pre-collision
-                // parent with spawned seeds (CollisionEvent.cc:125-133); the
-                // parent itself (k = nOut, outgoing pair 0) is updated last
-                // (CollisionEvent.cc:135).  One loop body keeps a single copy
-                // of updateTrajectory's sin/cos/log in the kernel.
 #pragma unroll 1
                 for (int k = 1; k <= nOut; k++) {
                     const bool parent = (k == nOut);
-                    Lane S = L;
-                    if (!parent) S.seed = rng_spawn(L.seed);
+                    const uint64_t seedK = parent ? L.seed : rng_spawn(L.seed);
                     const int j = parent ? 0 : k;
                     uint64_t t = s0;
                     double ek, ak;
@@ -1198,24 +1259,19 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                         ek = (20 * rr * rr);
                         ak = rng_sample(t) * 2.0 - 1.0;
                     }
-                    update_trajectory(S, ek, ak);
-                    if (parent) {
-                        L = S;
+                    if (nOut == 1) {
+                        // the survivor: updated at the top of the next iteration
+                        L.E = ek;
+                        L.nmfp = ak;
+                        upd = true;
+                        grp = true;
                     } else {
-                        uint32_t o = slot + (uint32_t)(k - 1);
-                        if (o < dcap) store_particle(dst, o, S, cell);
+                        const uint32_t o = slot + (uint32_t)(k - 1);
+                        if (o < dcap) store_deferred(dst, o, L, seedK, ek, ak, cell);
                         else atomicOr(io.error, 2u);
                     }
                 }
-                L.speed = vlength(L.vx, L.vy, L.vz);
-                if (nOut > 1) {
-                    uint32_t o = slot + (uint32_t)(nOut - 1);
-                    if (o < dcap) store_particle(dst, o, L, cell);
-                    else atomicOr(io.error, 2u);
-                    active = false;
-                } else {
-                    L.group = energy_group(T.energies, G, L.E);
-                }
+                if (nOut > 1) active = false;
             }
         } else if (outcome == 1) {
             // ---- MC_Facet_Crossing_Event (MC_Facet_Crossing_Event.cc:25-70) ----
@@ -1262,6 +1318,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         if (active && histSeg >= kMaxSegmentsPerHistory) {
             atomicOr(io.error, 8u);
             active = false;
+            upd = false;
+            grp = false;
         }
         }  // active
         cSeg += __popcll(__ballot(ev & kEvSegment));

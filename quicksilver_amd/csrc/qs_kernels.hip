@@ -630,7 +630,11 @@ struct Lane {
 };
 
 // updateTrajectory (CollisionEvent.cc:25-48)
-__device__ __forceinline__ void update_trajectory(Lane& L, double energy, double angle) {
+// updateTrajectory (CollisionEvent.cc:25-45) up to its last step, the new
+// number of mean free paths -1 * log(rngSample) (CollisionEvent.cc:44),
+// which the caller draws (sample_mfp) so that one copy of the log serves
+// every lane that needs a fresh draw.
+__device__ __forceinline__ void update_direction_energy(Lane& L, double energy, double angle) {
     L.E = energy;
     double cosTheta = angle;
     double r = rng_sample(L.seed);
@@ -644,7 +648,10 @@ __device__ __forceinline__ void update_trajectory(Lane& L, double energy, double
     L.vx = speed * L.a;
     L.vy = speed * L.b;
     L.vz = speed * L.g;
-    r = rng_sample(L.seed);
+}
+
+__device__ __forceinline__ void sample_mfp(Lane& L) {
+    const double r = rng_sample(L.seed);
     L.nmfp = -1.0 * log(r);
 }
 
@@ -973,7 +980,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     // held in L.E and L.nmfp) for a scatter survivor or a loaded collision
     // record, and the energy-group search of every loaded or updated
     // particle.  One copy of each serves all lanes that need it.
-    bool upd = false, rec = false, grp = false;
+    bool upd = false, rec = false, grp = false, ld = false;
     Lane L = {};
     uint32_t histSeg = 0;
     // wave-uniform (scalar) balance counters, updated from ballots at the
@@ -1027,32 +1034,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                     L.tc = src.tc()[idx];
                     L.nmfp = src.nmfp()[idx];
                     L.seed = src.seed()[idx];
-                    int cell = src.cell()[idx];
-                    rec = cell < 0;
-                    if (rec) cell = ~cell;
-                    L.ci = cell % P.nx;
-                    L.cj = (cell / P.nx) % P.ny;
-                    L.ck = cell / (P.nx * P.ny);
-                    if (rec) {
-                        // a collision output: its direction is in the
-                        // velocity slots until updateTrajectory runs
-                        L.a = L.vx;
-                        L.b = L.vy;
-                        L.g = L.vz;
-                        upd = true;
-                    } else {
-                        L.speed = vlength(L.vx, L.vy, L.vz);
-                        if (L.speed > 0) {
-                            double factor = 1.0 / L.speed;
-                            L.a = factor * L.vx;
-                            L.b = factor * L.vy;
-                            L.g = factor * L.vz;
-                        } else {
-                            L.a = L.b = L.g = 0.0;
-                        }
-                        if (L.tc <= 0.0) L.tc += P.dt;
-                    }
-                    grp = true;
+                    L.ci = src.cell()[idx];  // decoded after the refill
+                    ld = true;
                     histSeg = 0;
                     active = true;
                 }
@@ -1065,13 +1048,40 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         // room for this iteration's pushes (uniform: decided before any lane diverges)
         const bool stackOk = top + kStackMargin <= scap;
         if (__ballot(active) == 0) break;  // drained
-        // ---- updateTrajectory (CollisionEvent.cc:135-140) of scatter
-        // survivors and loaded collision records ----
-        if (upd) {
-            update_trajectory(L, L.E, L.nmfp);
-            L.speed = vlength(L.vx, L.vy, L.vz);
+        // ---- per-lane work of loaded particles and collision survivors,
+        // one copy for all lanes that need it ----
+        bool fresh = false;  // loaded particle starting without mean free paths
+        if (ld) {
+            // a collision output stored before its updateTrajectory has its
+            // cell complemented and its direction in the velocity slots
+            int cell = L.ci;
+            rec = cell < 0;
+            if (rec) cell = ~cell;
+            L.ci = cell % P.nx;
+            L.cj = (cell / P.nx) % P.ny;
+            L.ck = cell / (P.nx * P.ny);
             if (rec) {
-                // the record's store and reload (MC_Load_Particle)
+                L.a = L.vx;
+                L.b = L.vy;
+                L.g = L.vz;
+                upd = true;
+            } else {
+                // the first segment draws the mean free paths of a particle
+                // that has none (MC_Segment_Outcome.cc:65-69); drawn here, the
+                // same draw from the same seed, nothing drawn in between
+                fresh = L.nmfp == 0.0;
+            }
+            grp = true;
+        }
+        // updateTrajectory (CollisionEvent.cc:135-140) of scatter survivors
+        // and collision records
+        if (upd) update_direction_energy(L, L.E, L.nmfp);
+        if (upd || fresh) sample_mfp(L);
+        if (upd || ld) {
+            L.speed = vlength(L.vx, L.vy, L.vz);
+            if (ld) {
+                // MC_Load_Particle (MC_Load_Particle.cc:11-29); a record's
+                // store and reload
                 if (L.speed > 0) {
                     double factor = 1.0 / L.speed;
                     L.a = factor * L.vx;
@@ -1082,13 +1092,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 }
                 if (L.tc <= 0.0) L.tc += P.dt;
             }
-            upd = false;
-            rec = false;
         }
-        if (grp) {
-            L.group = energy_group(T.energies, G, L.E);
-            grp = false;
-        }
+        if (grp) L.group = energy_group(T.energies, G, L.E);
+        upd = rec = grp = ld = false;
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
         if (active) {

@@ -313,6 +313,12 @@ def main():
             "kernel": "k_track",
             "bytes_per_segment": bseg,
             "traffic_bytes_per_segment": traffic_seg,
+            "traffic_gbs": traffic_seg * stats["segments"] / kernel_s / 1e9 if traffic_seg and kernel_s > 0 else None,
+            "traffic_frac": traffic_seg * stats["segments"] / kernel_s / 1e9 / HBM_PEAK_GBS
+            if traffic_seg and kernel_s > 0 else None,
+            "note": "achieved uses the reference data model's bytes per segment; the kernel moves far fewer "
+                    "(traffic_bytes_per_segment, measured), so frac can exceed 1: HBM bandwidth is not what "
+                    "bounds it (traffic_frac is the measured share of the HBM peak)",
             "segments_per_launch": stats["segments"] / launches,
             "avg_launch_ms": kernel_s / launches * 1e3,
         },

@@ -205,6 +205,15 @@ int qsg_coral_check(qsg_problem p, qsg_coral_report* report);
  * decks that set coralBenchmark. */
 int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells);
 
+/* The tracking kernels' nearest-facet search (MCT_Nearest_Facet, MCT.cc:87-137
+ * with the retry of MCT.cc:487-536) on n given rays of problem p: xyz and dir
+ * hold [n][3] positions and direction cosines, cell the global cell ids
+ * (i + nx*(j + ny*k)).  Returns the distance to the nearest facet (clamped at
+ * 0), the facet (0..23) and the position after any nudge toward the cell
+ * centre.  A diagnostic: the device function is the one cycle tracking uses. */
+int qsg_nearest_facet_probe(qsg_problem p, uint64_t n, const double* xyz, const double* dir, const int32_t* cell,
+                            double* distance, int32_t* facet, double* xyz_out);
+
 /* Device FP64 builtins on host arrays (rounding checks).
  * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a) */
 int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device);

@@ -807,6 +807,28 @@ int qso_run_deck(const char* path, int cycles, uint64_t* bal13, double* flux, do
     return 0;
 }
 
+/* qso_nearest_facet on n rays of a deck's mesh (for ray-by-ray checks of a
+ * nearest-facet search): xyz, dir [n][3], cell [n] global ids; returns the
+ * distance, the facet and the position after any nudge.  The deck's tables
+ * are built once per call and released with the process. */
+int qso_nearest_facet_rays(const char* path, uint64_t n, const double* xyz, const double* dir, const int32_t* cell,
+                           double* dist, int32_t* facet, double* xyz_out) {
+    Run* R = (Run*)calloc(1, sizeof(Run));
+    if (qso_read_deck(path, &R->D)) { free(R); return 1; }
+    int rc = qso_build(&R->P, &R->D);
+    if (rc) { free(R); return 10 + rc; }
+    for (uint64_t q = 0; q < n; q++) {
+        if (cell[q] < 0 || cell[q] >= R->P.nc) return 2;
+        V3 r = {xyz[3 * q], xyz[3 * q + 1], xyz[3 * q + 2]};
+        V3 d = {dir[3 * q], dir[3 * q + 1], dir[3 * q + 2]};
+        int f = -1;
+        dist[q] = qso_nearest_facet(&R->P, cell[q], &r, d, &f);
+        facet[q] = f;
+        xyz_out[3 * q] = r.x; xyz_out[3 * q + 1] = r.y; xyz_out[3 * q + 2] = r.z;
+    }
+    return 0;
+}
+
 #ifdef QSO_MAIN
 int main(int argc, char** argv) {
     if (argc < 2) { fprintf(stderr, "usage: qs_oracle deck.inp [cycles]\n"); return 2; }

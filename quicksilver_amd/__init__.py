@@ -34,7 +34,7 @@ EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy"
            "qsg_batch_set_tuning", "qsg_batch_get_stats", "qsg_server_create", "qsg_server_destroy",
            "qsg_server_submit", "qsg_server_step", "qsg_server_status", "qsg_server_result",
            "qsg_server_release", "qsg_server_set_tuning", "qsg_server_get_stats",
-           "qsg_coral_check", "qsg_get_fluence"]
+           "qsg_coral_check", "qsg_get_fluence", "qsg_nearest_facet_probe"]
 
 
 class QsgError(RuntimeError):
@@ -214,6 +214,8 @@ def lib() -> ctypes.CDLL:
     L.qsg_get_processing_count.argtypes = [vp, _P(ctypes.c_uint64)]
     L.qsg_get_processing.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
     L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
+    _I32 = _P(ctypes.c_int32)
+    L.qsg_nearest_facet_probe.argtypes = [vp, ctypes.c_uint64, _D, _D, _I32, _D, _I32, _D]
     L.qsg_get_stats.argtypes = [vp, _P(Stats)]
     L.qsg_reset_stats.argtypes = [vp]
     L.qsg_batch_create.argtypes = [_P(vp), i, _P(vp)]
@@ -342,6 +344,25 @@ class Problem:
         out = np.zeros(self.info().n_cells, dtype=np.float64)
         _check(lib().qsg_get_fluence(self._h, out.ctypes.data_as(_D), out.size))
         return out
+
+    def nearest_facet(self, xyz: np.ndarray, direction: np.ndarray, cell: np.ndarray):
+        """The tracking kernels' nearest-facet search on given rays (see
+        qsg_nearest_facet_probe): returns (distance, facet, position after
+        any nudge)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        direction = np.ascontiguousarray(direction, dtype=np.float64).reshape(-1, 3)
+        cell = np.ascontiguousarray(cell, dtype=np.int32).reshape(-1)
+        n = cell.size
+        if xyz.shape[0] != n or direction.shape[0] != n:
+            raise ValueError("xyz, direction and cell must describe the same number of rays")
+        dist = np.empty(n, np.float64)
+        facet = np.empty(n, np.int32)
+        moved = np.empty((n, 3), np.float64)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        _check(lib().qsg_nearest_facet_probe(self._h, n, xyz.ctypes.data_as(_D), direction.ctypes.data_as(_D),
+                                             cell.ctypes.data_as(i32), dist.ctypes.data_as(_D),
+                                             facet.ctypes.data_as(i32), moved.ctypes.data_as(_D)))
+        return dist, facet, moved
 
     def coral_check(self) -> CoralReport:
         r = CoralReport()

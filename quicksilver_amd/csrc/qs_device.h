@@ -240,6 +240,11 @@ void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* 
                          double* fluxSums);
 
 // Device copies of single FP64 builtins, for rounding checks.
+// Nearest facet (with the nudge retry) of n rays: xyz, dir [n][3], cell [n]
+// (global ids, checked by the caller); xyzOut receives the possibly nudged
+// positions.
+void launchNearestProbe(const LaunchCtx& ctx, const DevProblem& P, const double* xyz, const double* dir,
+                        const int32_t* cell, double* dist, int32_t* facet, double* xyzOut, size_t n);
 void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out, size_t n);
 
 }  // namespace qs

@@ -295,18 +295,27 @@ struct FindNearest {
 // The four facets of the face of axis AX the lane leaves through.
 // The same test for a plane whose other two normal components are zero
 // (B = C = 0 for an x face, ...): its dot product is N*dir[AX] and its
-// numerator -(N*pos[AX] + D).  Dropping the zero terms changes no value:
-// B*y and C*z are signed zeros, and adding a signed zero to a nonzero sum
-// leaves it unchanged; when the sum is itself zero only the sign of a zero
-// distance can differ, and no comparison or product below can tell.
+// numerator -(N*pos[AX] + D).  Dropping the zero terms changes no nonzero
+// value: B*y and C*z are signed zeros, and adding a signed zero to a
+// nonzero partial sum leaves it unchanged.  Only a zero sum can differ, in
+// its sign (-0 + -0 is -0, but the reference's sum of four terms may hold a
+// +0); then the full plane row pf gives the reference's own four-term sum,
+// so even the sign of a zero distance is the reference's.
+template <int AX>
+__device__ __forceinline__ double plane_sum(double N, double D, const double* pf, double x, double y, double z) {
+    const double xa = AX == 0 ? x : (AX == 1 ? y : z);
+    double s = N * xa + D;
+    if (s == 0.0) s = pf[0] * x + pf[1] * y + pf[2] * z + pf[3];
+    return s;
+}
+
 template <int AX>
 __device__ __forceinline__ double distance_to_face_facet(double planeTol, double dot, double N, double D,
-                                                         double p0u, double p0v, double p1u, double p1v,
-                                                         double p2u, double p2v, double x, double y, double z,
-                                                         double a, double b, double g) {
+                                                         const double* pf, double p0u, double p0v, double p1u,
+                                                         double p1v, double p2u, double p2v, double x, double y,
+                                                         double z, double a, double b, double g) {
     const double tol = 1e-9;
-    const double xa = AX == 0 ? x : (AX == 1 ? y : z);
-    double numerator = -1.0 * (N * xa + D);
+    double numerator = -1.0 * plane_sum<AX>(N, D, pf, x, y, z);
     if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
     double distance = numerator / dot;
     double ix = x + distance * a;
@@ -331,9 +340,9 @@ __device__ __forceinline__ double distance_to_face_facet(double planeTol, double
 }
 
 template <int AX, int T>
-__device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
-                                            double x, double y, double z, double a, double b, double g, bool plus,
-                                            bool minus) {
+__device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pnd, const double* pl,
+                                            const Cell9& c9, double planeTol, double x, double y, double z,
+                                            double a, double b, double g, bool plus, bool minus) {
     constexpr int FP = 8 * AX + T, FM = FP + 4;
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
@@ -343,7 +352,7 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, c
         fn.add(kHuge, FM);
         return;
     }
-    const double* p = pl + 2 * f;
+    const double* p = pnd + 2 * f;
     const double N = p[0], D = p[1];
     const double dot = N * (AX == 0 ? a : (AX == 1 ? b : g));
     double d = kHuge;
@@ -351,7 +360,8 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, c
         const double p0u = node_coord<FP, FM, 0, U>(c9, plus), p0v = node_coord<FP, FM, 0, V>(c9, plus);
         const double p1u = node_coord<FP, FM, 1, U>(c9, plus), p1v = node_coord<FP, FM, 1, V>(c9, plus);
         const double p2u = node_coord<FP, FM, 2, U>(c9, plus), p2v = node_coord<FP, FM, 2, V>(c9, plus);
-        d = distance_to_face_facet<AX>(planeTol, dot, N, D, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a, b, g);
+        d = distance_to_face_facet<AX>(planeTol, dot, N, D, pl + 4 * f, p0u, p0v, p1u, p1v, p2u, p2v, x, y, z, a,
+                                       b, g);
     }
     // the opposite face faces away from the direction: distance kHuge
     fn.add(kHuge, plus ? FM : FP);
@@ -359,14 +369,15 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, c
 }
 
 template <int AX>
-__device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
-                                           double x, double y, double z, double a, double b, double g) {
+__device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pnd, const double* pl, const Cell9& c9,
+                                           double planeTol, double x, double y, double z, double a, double b,
+                                           double g) {
     const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
-    visit_facet<AX, 0>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 1>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 2>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 3>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 0>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 1>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 2>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 3>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
 }
 
 // One facet of any face, for the rare retry path: facets are visited in
@@ -478,8 +489,9 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
 // per lane at run time: the projection axes are selected per lane, so the
 // whole wave runs one copy of the test whatever face each lane's candidate
 // lies on.
-__device__ __forceinline__ double exact_facet_test(int f, const double* pnd, const Cell9& c9, double planeTol,
-                                                   double x, double y, double z, double a, double b, double g) {
+__device__ __forceinline__ double exact_facet_test(int f, const double* pnd, const double* pl, const Cell9& c9,
+                                                   double planeTol, double x, double y, double z, double a,
+                                                   double b, double g) {
     const int ax = f >> 3;
     const double N = pnd[2 * f], D = pnd[2 * f + 1];
     const double dAx = ax == 0 ? a : (ax == 1 ? b : g);
@@ -487,7 +499,13 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pnd, con
     if (!(dot > 0.0)) return kHuge;
     const double tol = 1e-9;
     const double xa = ax == 0 ? x : (ax == 1 ? y : z);
-    double numerator = -1.0 * (N * xa + D);
+    double sum = N * xa + D;
+    if (sum == 0.0) {
+        // a zero sum: the sign of zero of the reference's four-term sum
+        const double* pf = pl + 4 * f;
+        sum = pf[0] * x + pf[1] * y + pf[2] * z + pf[3];
+    }
+    double numerator = -1.0 * sum;
     if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
     double distance = numerator / dot;
     double ix = x + distance * a;
@@ -519,11 +537,14 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pnd, con
 // MCT_Nearest_Facet + MCT_Nearest_Facet_3D_G (MCT.cc:87-137, 550-621) with
 // the retry of MCT_Nearest_Facet_Find_Nearest (MCT.cc:487-536): when no
 // facet bounds the particle it is nudged toward the cell centre and the
-// search repeats.  The common first search tests only the 12 facets of the
-// three faces the direction points out of (the other 12 have dot <= 0 and
-// distance kHuge in the reference too), fully unrolled; the rare retries
-// use a compact loop over all 24 facets so they add no register pressure
-// to the common path.
+// search repeats.  The first search considers only the three faces the
+// direction points out of (the other 12 facets have dot <= 0 and distance
+// kHuge in the reference too).  A probe of each of those faces names the one
+// or two triangles that can hold the exit point; the lane's candidates (at
+// most three) get the reference's exact test in facet order, and every
+// other facet is kHuge.  A lane whose probe cannot decide runs the exact
+// test on all 12 exit-face facets.  The rare retries use a compact loop
+// over all 24 facets so they add no register pressure to the common path.
 __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, const double* pnd,
                                                  int cell, const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
@@ -549,9 +570,9 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
         }
         if (f0.unsure || f1.unsure || f2.unsure || n > 3) {
             // the reference's test on the 12 facets of the three exit faces
-            visit_axis<0>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-            visit_axis<1>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
-            visit_axis<2>(fn, pnd, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<0>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<1>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<2>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
         } else {
             // exact tests of the candidates, merged in facet order; every
             // other facet is kHuge, which only sets the largest kHuge index
@@ -559,7 +580,7 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
 #pragma unroll 1
             for (int i = 0; i < n; i++) {
                 const int f = i == 0 ? q0 : (i == 1 ? q1 : q2);
-                const double d = exact_facet_test(f, pnd, c9, planeTol, x, y, z, a, b, g);
+                const double d = exact_facet_test(f, pnd, pl, c9, planeTol, x, y, z, a, b, g);
                 fn.add(d, f);
                 const bool fin = d != kHuge;
                 if (i == 0) fin0 = fin;
@@ -1669,6 +1690,29 @@ __global__ void k_math(int op, const double* a, const double* b, double* out, si
     out[q] = r;
 }
 
+// The tracking kernels' nearest-facet search on given rays, one ray per
+// thread: the same device function, so its exit-face shortcut can be checked
+// ray by ray against the reference's full 24-facet scan.
+__global__ void __launch_bounds__(256) k_nearest_probe(DevProblem P, const double* __restrict__ xyz,
+                                                       const double* __restrict__ dir,
+                                                       const int32_t* __restrict__ cell, double* __restrict__ dist,
+                                                       int32_t* __restrict__ facet, double* __restrict__ xyzOut,
+                                                       size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int c = cell[q];
+    const int ci = c % P.nx, cj = (c / P.nx) % P.ny, ck = c / (P.nx * P.ny);
+    double x = xyz[3 * q], y = xyz[3 * q + 1], z = xyz[3 * q + 2];
+    const double a = dir[3 * q], b = dir[3 * q + 1], g = dir[3 * q + 2];
+    const Cell9 c9 = cell_nodes(P, ci, cj, ck);
+    const Nearest nf = nearest_facet(P, P.planes + (size_t)c * 96, P.planeND + (size_t)c * 48, c, c9, x, y, z, a, b, g);
+    dist[q] = nf.d;
+    facet[q] = nf.f;
+    xyzOut[3 * q] = x;
+    xyzOut[3 * q + 1] = y;
+    xyzOut[3 * q + 2] = z;
+}
+
 // Fluence::compute (Tallies.cc:100-120): per cell, the groups' flux added
 // one by one to the cumulative value.
 __global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux, double* __restrict__ fluence,
@@ -2214,6 +2258,14 @@ void launchFluence(const LaunchCtx& ctx, const DevProblem& P) {
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fluence, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, P.flux, P.fluence, P.nCells,
                        P.G);
+}
+
+void launchNearestProbe(const LaunchCtx& ctx, const DevProblem& P, const double* xyz, const double* dir,
+                        const int32_t* cell, double* dist, int32_t* facet, double* xyzOut, size_t n) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_nearest_probe, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, P, xyz, dir, cell,
+                       dist, facet, xyzOut, n);
 }
 
 void launchMathProbe(const LaunchCtx& ctx, int op, const double* a, const double* b, double* out,

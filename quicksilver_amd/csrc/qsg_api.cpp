@@ -726,6 +726,48 @@ int qsg_coral_check(qsg_problem p, qsg_coral_report* r) {
     })
 }
 
+int qsg_nearest_facet_probe(qsg_problem p, uint64_t n, const double* xyz, const double* dir, const int32_t* cell,
+                            double* distance, int32_t* facet, double* xyz_out) {
+    QSG_GUARD({
+        if (!p || !xyz || !dir || !cell || !distance || !facet || !xyz_out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (n > (1ull << 26)) throw QsgError(QSG_ERR_ARG, "at most 2^26 rays per call");
+        for (uint64_t q = 0; q < n; q++)
+            if (cell[q] < 0 || cell[q] >= p->tables.nCells) throw QsgError(QSG_ERR_ARG, "cell id outside the mesh");
+        if (n == 0) return QSG_OK;
+        setDevice(p);
+        const size_t v3 = (size_t)n * 3 * sizeof(double);
+        char* m = nullptr;
+        // one scratch allocation: xyz, dir, xyzOut, distance, cell, facet
+        const size_t bytes = 3 * v3 + (size_t)n * (sizeof(double) + 2 * sizeof(int32_t));
+        hipCheck(hipMalloc(&m, bytes), "hipMalloc");
+        double* dXyz = (double*)m;
+        double* dDir = (double*)(m + v3);
+        double* dOut = (double*)(m + 2 * v3);
+        double* dDist = (double*)(m + 3 * v3);
+        int32_t* dCell = (int32_t*)(m + 3 * v3 + (size_t)n * sizeof(double));
+        int32_t* dFacet = dCell + n;
+        try {
+            hipCheck(hipMemcpyAsync(dXyz, xyz, v3, hipMemcpyHostToDevice, p->stream), "hipMemcpyAsync");
+            hipCheck(hipMemcpyAsync(dDir, dir, v3, hipMemcpyHostToDevice, p->stream), "hipMemcpyAsync");
+            hipCheck(hipMemcpyAsync(dCell, cell, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, p->stream),
+                     "hipMemcpyAsync");
+            qs::launchNearestProbe(p->ctx(), p->dp, dXyz, dDir, dCell, dDist, dFacet, dOut, n);
+            hipCheck(hipGetLastError(), "k_nearest_probe launch");
+            hipCheck(hipMemcpyAsync(distance, dDist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, p->stream),
+                     "hipMemcpyAsync");
+            hipCheck(hipMemcpyAsync(facet, dFacet, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream),
+                     "hipMemcpyAsync");
+            hipCheck(hipMemcpyAsync(xyz_out, dOut, v3, hipMemcpyDeviceToHost, p->stream), "hipMemcpyAsync");
+            hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        } catch (...) {
+            (void)hipStreamSynchronize(p->stream);
+            (void)hipFree(m);
+            throw;
+        }
+        hipCheck(hipFree(m), "hipFree");
+    })
+}
+
 int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device) {
     QSG_GUARD({
         if (!a || !b || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");

@@ -13,6 +13,8 @@
 //   k_flux_*       deterministic flux sum + reset       Tallies.cc:95,146-171
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cmath>
 #include <cstdint>
 
@@ -2084,6 +2086,8 @@ hipError_t ensureCodes(hipStream_t s) {
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    static std::mutex m;  // problems may be built from several host threads
+    std::lock_guard<std::mutex> lock(m);
     if (g_codesReady[dev]) return hipSuccess;
     uint32_t code[24];
     facetNodeCodes(code);

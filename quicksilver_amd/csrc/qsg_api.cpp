@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -102,7 +104,7 @@ void checkErrors(qsg_problem_s* p) {
 void build(qsg_problem_s* p, int device) {
     p->device = device;
     setDevice(p);
-    hipCheck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    p->stream = qsgi::acquireStream(device);
     hipCheck(hipDeviceGetAttribute(&p->numCUs, hipDeviceAttributeMultiprocessorCount, device),
              "hipDeviceGetAttribute");
     for (hipEvent_t* e : {&p->ev0, &p->ev1, &p->evI0, &p->evI1, &p->evF0, &p->evF1})
@@ -196,13 +198,18 @@ void build(qsg_problem_s* p, int device) {
     p->dc = p->dalloc<qs::DevCounters>(1);
     hipCheck(hipMemset(p->pc, 0, sizeof(qs::PassCounters)), "hipMemset");
     hipCheck(hipMemset(p->dc, 0, sizeof(qs::DevCounters)), "hipMemset");
-    hipCheck(hipHostMalloc((void**)&p->hpc, sizeof(qs::PassCounters), hipHostMallocDefault), "hipHostMalloc");
-    hipCheck(hipHostMalloc((void**)&p->hdc, sizeof(qs::DevCounters), hipHostMallocDefault), "hipHostMalloc");
-    hipCheck(hipHostMalloc((void**)&p->hFluxSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    static_assert(sizeof(qs::PassCounters) <= qsgi::kPinnedBlock && sizeof(qs::DevCounters) <= qsgi::kPinnedBlock,
+                  "counter blocks must fit a pinned block");
+    p->hpc = static_cast<qs::PassCounters*>(qsgi::acquirePinned(sizeof(qs::PassCounters)));
+    p->hdc = static_cast<qs::DevCounters*>(qsgi::acquirePinned(sizeof(qs::DevCounters)));
+    p->hFluxSum = static_cast<double*>(qsgi::acquirePinned(sizeof(double)));
     p->nPartials = 1024;
     p->partials = p->dalloc<double>(p->nPartials);
     p->fluxSumDev = p->dalloc<double>(1);
-    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    // the uploads and clears above ran on the null stream, which the
+    // problem's non-blocking stream does not wait for; other devices' and
+    // other problems' work is not waited for
+    hipCheck(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
 }
 
 void destroyProblem(qsg_problem_s* p) {
@@ -211,13 +218,13 @@ void destroyProblem(qsg_problem_s* p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void* a : p->allocs) (void)hipFree(a);
     p->stacks.release();
-    if (p->hpc) (void)hipHostFree(p->hpc);
-    if (p->hdc) (void)hipHostFree(p->hdc);
-    if (p->hFluxSum) (void)hipHostFree(p->hFluxSum);
+    qsgi::releasePinned(p->hpc);
+    qsgi::releasePinned(p->hdc);
+    qsgi::releasePinned(p->hFluxSum);
     for (hipEvent_t e : {p->ev0, p->ev1, p->evI0, p->evI1, p->evF0, p->evF1})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->passEv) (void)hipEventDestroy(e);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->stream) qsgi::releaseStream(p->device, p->stream);
     delete p;
 }
 
@@ -393,6 +400,55 @@ qs::WaveStacks qsgi::StackPool::ensure(uint32_t nWaves, uint32_t cap, hipStream_
         bytes = need;
     }
     return qs::WaveStacks{mem, cap, nWaves};
+}
+
+namespace {
+std::mutex g_poolMutex;
+std::map<int, std::vector<hipStream_t>> g_streams;  // idle streams per device
+std::vector<char*> g_pinned;                        // idle pinned blocks
+}  // namespace
+
+hipStream_t qsgi::acquireStream(int device) {
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        std::vector<hipStream_t>& v = g_streams[device];
+        if (!v.empty()) {
+            hipStream_t s = v.back();
+            v.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    return s;
+}
+
+void qsgi::releaseStream(int device, hipStream_t s) {
+    if (!s) return;
+    std::lock_guard<std::mutex> lock(g_poolMutex);
+    g_streams[device].push_back(s);
+}
+
+void* qsgi::acquirePinned(size_t bytes) {
+    if (bytes > kPinnedBlock) throw QsgError(QSG_ERR_ARG, "pinned block request too large");
+    std::lock_guard<std::mutex> lock(g_poolMutex);
+    if (g_pinned.empty()) {
+        // one pinned chunk, carved into blocks
+        constexpr size_t kChunk = 64 * kPinnedBlock;
+        void* m = nullptr;
+        hipCheck(hipHostMalloc(&m, kChunk, hipHostMallocDefault), "hipHostMalloc");
+        for (size_t o = 0; o < kChunk; o += kPinnedBlock) g_pinned.push_back(static_cast<char*>(m) + o);
+    }
+    char* b = g_pinned.back();
+    g_pinned.pop_back();
+    std::memset(b, 0, kPinnedBlock);
+    return b;
+}
+
+void qsgi::releasePinned(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_poolMutex);
+    g_pinned.push_back(static_cast<char*>(p));
 }
 
 void qsgi::StackPool::release() {

@@ -82,7 +82,10 @@ void destroyBatch(qsg_batch_s* b) {
     for (hipGraphExec_t g : {b->gInit, b->gFinal})
         if (g) (void)hipGraphExecDestroy(g);
     b->ownStacks.release();
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        qsgi::releaseStream(b->device, b->stream);
+    }
     delete b;
 }
 
@@ -169,7 +172,7 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, int capacity, int device,
             checkMember(b, problems[i]);
         }
         hipCheck(hipSetDevice(b->device), "hipSetDevice");
-        hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        b->stream = qsgi::acquireStream(b->device);
         hipCheck(hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, b->device),
                  "hipDeviceGetAttribute");
         hipCheck(hipEventCreate(&b->ev0), "hipEventCreate");

@@ -49,6 +49,17 @@ inline void addBalance(qsg_balance& acc, const qsg_balance& b) {
     acc.end += b.end;
 }
 
+// Streams and small pinned host blocks are handed back to process-wide free
+// lists when a problem is released and reused by the next one: creating and
+// destroying a stream, or pinning and unpinning host memory, costs
+// milliseconds, more than building a small problem.  Thread-safe; what the
+// lists hold lives until the process ends.
+hipStream_t acquireStream(int device);   // non-blocking stream on `device` (current)
+void releaseStream(int device, hipStream_t s);  // s must be idle
+constexpr size_t kPinnedBlock = 256;     // bytes per pinned block
+void* acquirePinned(size_t bytes);       // bytes <= kPinnedBlock
+void releasePinned(void* p);
+
 // Device memory for the per-wave secondary stacks of tracking launches,
 // grown on demand; the owner's stream is synchronised before memory that a
 // launch may still use is freed.

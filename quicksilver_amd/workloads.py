@@ -11,6 +11,7 @@
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -19,23 +20,59 @@ import quicksilver_amd as q
 from quicksilver_amd.decks import Deck, Request
 
 
+BUILD_THREADS = 8
+
+
+def build_problems(texts: Sequence[str], tuning: Optional[Dict[str, int]] = None,
+                   threads: int = BUILD_THREADS) -> List[q.Problem]:
+    """Constructs one problem per deck on `threads` host threads (the C ABI
+    builds independent problems concurrently, and ctypes releases the GIL
+    for each call).  On any failure every problem built is released."""
+    q.lib()
+
+    def one(text: str) -> q.Problem:
+        p = q.Problem(text)
+        if tuning:
+            p.set_tuning(**tuning)
+        return p
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        futures = [ex.submit(one, t) for t in texts]
+    probs, err = [], None
+    for f in futures:
+        try:
+            probs.append(f.result())
+        except Exception as e:  # noqa: BLE001 -- re-raised below
+            err = err or e
+    if err is not None:
+        for p in probs:
+            p.close()
+        raise err
+    return probs
+
+
 def run_one_at_a_time(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None) -> Dict:
-    """Each deck alone: construct, run all its steps, release.  solve_s
-    counts only the steps (what the batched form is compared on); wall_s
-    also counts construction and release."""
+    """Each deck alone: every step of one problem, then the next.  The
+    problems are constructed up front, the same way as for run_batched, and
+    each is released after its run.  solve_s counts only the steps (what the
+    batched form is compared on); wall_s also counts construction and
+    release."""
     t0 = time.perf_counter()
     segments = 0
     solve = 0.0
     per = []
-    for d in decks:
-        with q.Problem(d.text()) as p:
-            if tuning:
-                p.set_tuning(**tuning)
+    probs = build_problems([d.text() for d in decks], tuning)
+    try:
+        for p in probs:
             s0 = time.perf_counter()
             out = p.run()
             solve += time.perf_counter() - s0
-        per.append([b for b, _, _ in out])
-        segments += sum(b["num_segments"] for b, _, _ in out)
+            p.close()
+            per.append([b for b, _, _ in out])
+            segments += sum(b["num_segments"] for b, _, _ in out)
+    finally:
+        for p in probs:
+            p.close()
     wall = time.perf_counter() - t0
     return {"segments": segments, "wall_s": wall, "solve_s": solve, "segments_per_s": segments / solve,
             "end_to_end_segments_per_s": segments / wall, "balances": per}
@@ -44,7 +81,7 @@ def run_one_at_a_time(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = 
 def run_batched(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None) -> Dict:
     """All decks in one batch; solve_s counts the batch steps only."""
     t0 = time.perf_counter()
-    probs = [q.Problem(d.text()) for d in decks]
+    probs = build_problems([d.text() for d in decks])
     try:
         with q.Batch(probs) as b:
             if tuning:

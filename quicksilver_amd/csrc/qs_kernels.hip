@@ -415,10 +415,13 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
 // returns kHuge.  The probe names those candidates (one, or two near a
 // diagonal; none when the point is outside the face) or, when it cannot
 // decide -- particle on or behind the plane, a near-grazing direction, a
-// point near the face centre -- says so.
+// point near the face centre -- says so.  A lone candidate whose region holds
+// the point deeper than margin from every edge of the triangle is "deep":
+// the reference's containment test passes for it.
 struct FaceProbe {
     int c1, c2;  // candidate facet indices, c1 < c2, or -1
     bool unsure;
+    bool deep;
 };
 
 template <int AX>
@@ -426,7 +429,7 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
                                                      double z, double a, double b, double g) {
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
-    FaceProbe r = {-1, -1, false};
+    FaceProbe r = {-1, -1, false, false};
     const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
     if (!(plus || minus)) return r;  // parallel to both faces: dot = 0, all kHuge
@@ -478,6 +481,9 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
         else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
         const int t1 = kEdgeTri.tri[AX][side][wedge];
         r.c1 = base + t1;
+        // no second candidate: farther than margin from both diagonals;
+        // and farther than margin inside the face's own edges
+        r.deep = other < 0 && fabs(du) <= (hu - margin) * dot0 && fabs(dv) <= (hv - margin) * dot0;
         if (other >= 0) {
             const int t2 = kEdgeTri.tri[AX][side][other];
             r.c1 = base + (t1 < t2 ? t1 : t2);
@@ -485,6 +491,30 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
         }
     }
     return r;
+}
+
+// The reference's distance to facet f's plane (distance_to_face_facet up
+// to its containment test), for a facet chosen per lane at run time: kHuge
+// when the direction does not leave through the plane or the particle lies
+// behind it beyond the plane tolerance.
+__device__ __forceinline__ double facet_plane_distance(int f, const double* pnd, const double* pl, double planeTol,
+                                                       double x, double y, double z, double a, double b,
+                                                       double g) {
+    const int ax = f >> 3;
+    const double N = pnd[2 * f], D = pnd[2 * f + 1];
+    const double dAx = ax == 0 ? a : (ax == 1 ? b : g);
+    const double dot = N * dAx;
+    if (!(dot > 0.0)) return kHuge;
+    const double xa = ax == 0 ? x : (ax == 1 ? y : z);
+    double sum = N * xa + D;
+    if (sum == 0.0) {
+        // a zero sum: the sign of zero of the reference's four-term sum
+        const double* pf = pl + 4 * f;
+        sum = pf[0] * x + pf[1] * y + pf[2] * z + pf[3];
+    }
+    double numerator = -1.0 * sum;
+    if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
+    return numerator / dot;
 }
 
 // The reference's exact test (distance_to_face_facet) for a facet chosen
@@ -495,21 +525,9 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pnd, con
                                                    double planeTol, double x, double y, double z, double a,
                                                    double b, double g) {
     const int ax = f >> 3;
-    const double N = pnd[2 * f], D = pnd[2 * f + 1];
-    const double dAx = ax == 0 ? a : (ax == 1 ? b : g);
-    const double dot = N * dAx;
-    if (!(dot > 0.0)) return kHuge;
     const double tol = 1e-9;
-    const double xa = ax == 0 ? x : (ax == 1 ? y : z);
-    double sum = N * xa + D;
-    if (sum == 0.0) {
-        // a zero sum: the sign of zero of the reference's four-term sum
-        const double* pf = pl + 4 * f;
-        sum = pf[0] * x + pf[1] * y + pf[2] * z + pf[3];
-    }
-    double numerator = -1.0 * sum;
-    if (numerator < 0.0 && numerator * numerator > planeTol) return kHuge;
-    double distance = numerator / dot;
+    const double distance = facet_plane_distance(f, pnd, pl, planeTol, x, y, z, a, b, g);
+    if (distance == kHuge) return kHuge;
     double ix = x + distance * a;
     double iy = y + distance * b;
     double iz = z + distance * g;
@@ -570,7 +588,16 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
                 n++;
             }
         }
-        if (f0.unsure || f1.unsure || f2.unsure || n > 3) {
+        const bool unsure = f0.unsure || f1.unsure || f2.unsure;
+        const bool deep = !unsure && n == 1 && (f0.c1 >= 0 ? f0.deep : (f1.c1 >= 0 ? f1.deep : f2.deep));
+        if (deep) {
+            // the only candidate, its exit point deeper inside it than
+            // margin: the reference's containment test passes for it, so
+            // only the distance is computed
+            const double d = facet_plane_distance(q0, pnd, pl, planeTol, x, y, z, a, b, g);
+            fn.add(d, q0);
+            fn.add(kHuge, (d != kHuge && q0 == 23) ? 22 : 23);
+        } else if (unsure || n > 3) {
             // the reference's test on the 12 facets of the three exit faces
             visit_axis<0>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
             visit_axis<1>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);

@@ -75,6 +75,7 @@ struct DevProblem {
     int nMat, maxR;
     const double* energies;  // [G+1]
     const double* total;     // [nMat][G]
+    const double* mfp;       // [nMat][G] mean free path 1/total (1e75 where total is 0)
     const double* macro;     // [nMat][maxR][G]
     const int32_t* reactType;  // [nMat][maxR]
     MatInfo mat[kMaxMaterials];

@@ -407,9 +407,9 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
 }
 
 // Probe of one exit face.  The four triangles of a face lie in the face
-// plane (their stored planes differ only in the last bits), so the exact
-// intersection point with any of them is within ~1e-15 of the probe's, the
-// intersection with triangle 0's plane.  Only a triangle whose region,
+// plane (their stored planes differ from it only in the last bits), so the
+// exact intersection point with any of them is within ~1e-15 of the probe's,
+// the intersection with the face plane.  Only a triangle whose region,
 // widened by `margin` (far above the reference's 1e-9 tolerances), holds the
 // probe point can pass the reference's containment test; every other one
 // returns kHuge.  The probe names those candidates (one, or two near a
@@ -425,7 +425,7 @@ struct FaceProbe {
 };
 
 template <int AX>
-__device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Cell9& c9, double x, double y,
+__device__ __forceinline__ FaceProbe probe_exit_face(const Cell9& c9, double x, double y,
                                                      double z, double a, double b, double g) {
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
@@ -434,10 +434,14 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
     if (!(plus || minus)) return r;  // parallel to both faces: dot = 0, all kHuge
     const int base = 8 * AX + (plus ? 0 : 4);
+    // The face is the plane coordinate = Xf; the stored planes of its four
+    // triangles equal it to the last bits, so the probe needs no plane loads.
+    // dot0 > 0 and num0 are the direction and distance along the axis.
     const double xa = AX == 0 ? x : (AX == 1 ? y : z);
-    const double N0 = pnd[2 * base], D0 = pnd[2 * base + 1];
-    const double dot0 = N0 * dAx;
-    const double num0 = -1.0 * (N0 * xa + D0);
+    const double Xf = plus ? (AX == 0 ? c9.X1 : (AX == 1 ? c9.Y1 : c9.Z1))
+                           : (AX == 0 ? c9.X0 : (AX == 1 ? c9.Y0 : c9.Z0));
+    const double dot0 = plus ? dAx : -dAx;
+    const double num0 = plus ? Xf - xa : xa - Xf;
     const double U0 = AX == 0 ? c9.Y0 : (AX == 1 ? c9.Z0 : c9.X0);
     const double U1 = AX == 0 ? c9.Y1 : (AX == 1 ? c9.Z1 : c9.X1);
     const double Uh = AX == 0 ? c9.Yh : (AX == 1 ? c9.Zh : c9.Xh);
@@ -446,9 +450,10 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const double* pnd, const Ce
     const double Vh = AX == 0 ? c9.Zh : (AX == 1 ? c9.Xh : c9.Yh);
     const double hu = 0.5 * (U1 - U0), hv = 0.5 * (V1 - V0);
     const double margin = fmax(1e-6 * fmin(hu, hv), 1e-7);
-    // bound on how far the four numerators can differ (last-bit differences
-    // of the planes, times the coordinates involved)
-    const double errN = 4e-16 * (fabs(xa) + fabs(D0)) + 1e-300;
+    // bound on how far num0 and the four triangles' numerators (scaled to
+    // this axis) can differ: last-bit differences of the planes and of the
+    // subtraction, times the coordinates involved
+    const double errN = 8e-16 * (fabs(xa) + fabs(Xf)) + 1e-300;
     if (!(num0 > 1e3 * errN) || !(errN < 1e-3 * margin * dot0)) {
         r.unsure = true;
         return r;
@@ -572,9 +577,9 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
         FindNearest fn = {1e80, -1, -1, -kHuge, 0};
-        const FaceProbe f0 = probe_exit_face<0>(pnd, c9, x, y, z, a, b, g);
-        const FaceProbe f1 = probe_exit_face<1>(pnd, c9, x, y, z, a, b, g);
-        const FaceProbe f2 = probe_exit_face<2>(pnd, c9, x, y, z, a, b, g);
+        const FaceProbe f0 = probe_exit_face<0>(c9, x, y, z, a, b, g);
+        const FaceProbe f1 = probe_exit_face<1>(c9, x, y, z, a, b, g);
+        const FaceProbe f2 = probe_exit_face<2>(c9, x, y, z, a, b, g);
         // the lane's candidates in facet order (the faces are probed in
         // facet order, each face's candidates ascending), at most three
         int n = 0, q0 = -1, q1 = -1, q2 = -1;
@@ -795,6 +800,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 struct Tables {
     const double* energies;
     const double* total;
+    const double* mfp;
     const double* macro;
     const int32_t* reactType;
     const MatInfo* mat;
@@ -806,6 +812,7 @@ __device__ __forceinline__ Tables stage_tables(const DevProblem& P, double* lds)
     if (!kLds) {
         t.energies = P.energies;
         t.total = P.total;
+        t.mfp = P.mfp;
         t.macro = P.macro;
         t.reactType = P.reactType;
         t.mat = P.mat;  // not reached: the global variant copies MatInfo to LDS too
@@ -814,12 +821,15 @@ __device__ __forceinline__ Tables stage_tables(const DevProblem& P, double* lds)
     const int nE = P.G + 1, nT = P.nMat * P.G, nM = P.nMat * P.maxR * P.G;
     double* sE = lds;
     double* sT = sE + nE;
-    double* sM = sT + nT;
+    double* sF = sT + nT;
+    double* sM = sF + nT;
     for (int q = threadIdx.x; q < nE; q += blockDim.x) sE[q] = P.energies[q];
     for (int q = threadIdx.x; q < nT; q += blockDim.x) sT[q] = P.total[q];
+    for (int q = threadIdx.x; q < nT; q += blockDim.x) sF[q] = P.mfp[q];
     for (int q = threadIdx.x; q < nM; q += blockDim.x) sM[q] = P.macro[q];
     t.energies = sE;
     t.total = sT;
+    t.mfp = sF;
     t.macro = sM;
     t.reactType = P.reactType;
     t.mat = P.mat;
@@ -902,6 +912,7 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.maxR = rfl(c.maxR);
     P.energies = rflp(c.energies);
     P.total = rflp(c.total);
+    P.mfp = rflp(c.mfp);
     P.macro = rflp(c.macro);
     P.reactType = rflp(c.reactType);
     P.dt = rfld(c.dt);
@@ -939,6 +950,7 @@ struct BatchQueue {
         Tables T;
         T.energies = rflp(P.energies);
         T.total = rflp(P.total);
+        T.mfp = rflp(P.mfp);
         T.macro = rflp(P.macro);
         T.reactType = sReact;
         T.mat = sMat;
@@ -1157,7 +1169,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             L.nmfp = kSmall;
         }
         const double sigt = T.total[mat * G + L.group];
-        const double mfp = (sigt == 0.0) ? kHuge : 1.0 / sigt;
+        const double mfp = T.mfp[mat * G + L.group];  // (sigt == 0.0) ? kHuge : 1.0 / sigt
         if (L.nmfp == 0.0) {
             double r = rng_sample(L.seed);
             L.nmfp = -1.0 * log(r);
@@ -2129,7 +2141,7 @@ hipError_t ensureCodes(hipStream_t s) {
 int prepareDevice(const LaunchCtx& ctx) { return (int)ensureCodes((hipStream_t)ctx.stream); }
 
 size_t tableLdsBytes(const DevProblem& P) {
-    return sizeof(double) * ((size_t)(P.G + 1) + (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);
+    return sizeof(double) * ((size_t)(P.G + 1) + 2 * (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);
 }
 
 void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, uint32_t outBase,

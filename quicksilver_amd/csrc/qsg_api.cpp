@@ -153,11 +153,18 @@ void build(qsg_problem_s* p, int device) {
     d.nMat = (int)T.materials.size();
     d.maxR = T.maxReactions;
     d.energies = p->upload(T.energies);
-    std::vector<double> total((size_t)d.nMat * d.G), macro((size_t)d.nMat * d.maxR * d.G, 0.0);
+    std::vector<double> total((size_t)d.nMat * d.G), mfp((size_t)d.nMat * d.G),
+        macro((size_t)d.nMat * d.maxR * d.G, 0.0);
     std::vector<int32_t> react((size_t)d.nMat * d.maxR, 0);
     for (int m = 0; m < d.nMat; m++) {
         const qs::MaterialTables& mt = T.materials[m];
-        for (int g = 0; g < d.G; g++) total[(size_t)m * d.G + g] = mt.total[g];
+        for (int g = 0; g < d.G; g++) {
+            // MC_Segment_Outcome.cc:60-63: the mean free path, one IEEE
+            // division per (material, group) instead of one per segment
+            const double t = mt.total[g];
+            total[(size_t)m * d.G + g] = t;
+            mfp[(size_t)m * d.G + g] = (t == 0.0) ? 1.0e+75 : 1.0 / t;
+        }
         for (int r = 0; r < mt.nReactions; r++) {
             react[(size_t)m * d.maxR + r] = mt.reactionType[r];
             for (int g = 0; g < d.G; g++)
@@ -169,6 +176,7 @@ void build(qsg_problem_s* p, int device) {
         d.mat[m].nuBar = mt.nuBar;
     }
     d.total = p->upload(total);
+    d.mfp = p->upload(mfp);
     d.macro = p->upload(macro);
     d.reactType = p->upload(react);
     d.dt = T.dt;

@@ -61,6 +61,7 @@ constexpr int kMaxReactions = 16;
 struct DevProblem {
     // grid
     int nx, ny, nz, nCells;
+    double rNx, rNxy;      // 1/nx and 1/(nx*ny), for decoding cell ids
     double dx, dy, dz;     // lx/nx ...
     double hdx, hdy, hdz;  // dx/2.0 ...
     int boundaryMask;      // 2 bits per face +x -x +y -y +z -z: 1 escape, 2 reflect

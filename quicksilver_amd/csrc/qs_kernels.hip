@@ -98,6 +98,36 @@ struct Cell9 {
     double X0, X1, Xh, Y0, Y1, Yh, Z0, Z1, Zh;
 };
 
+// Cell id i + nx*(j + ny*k) (the reference's global cell index with one
+// domain) to (i, j, k).  The quotients come from a double product with the
+// host's reciprocal, within 2^-21 of the true quotient for ids under 2^31,
+// so the truncated value is off by at most one either way; the remainder
+// test corrects it.  Integer division has no instruction here.
+__device__ __forceinline__ void decode_cell(const DevProblem& P, int cell, int& i, int& j, int& k) {
+    const int nxy = P.nx * P.ny;
+    int kq = (int)((double)cell * P.rNxy);
+    int r = cell - kq * nxy;
+    if (r < 0) {
+        kq--;
+        r += nxy;
+    } else if (r >= nxy) {
+        kq++;
+        r -= nxy;
+    }
+    int jq = (int)((double)r * P.rNx);
+    int iq = r - jq * P.nx;
+    if (iq < 0) {
+        jq--;
+        iq += P.nx;
+    } else if (iq >= P.nx) {
+        jq++;
+        iq -= P.nx;
+    }
+    i = iq;
+    j = jq;
+    k = kq;
+}
+
 __device__ __forceinline__ Cell9 cell_nodes(const DevProblem& P, int i, int j, int k) {
     Cell9 c;
     c.X0 = (double)i * P.dx;
@@ -891,6 +921,8 @@ __device__ __forceinline__ Particles rflv(const Particles& v) { return Particles
 __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     DevProblem P;
     P.nx = rfl(c.nx);
+    P.rNx = rfld(c.rNx);
+    P.rNxy = rfld(c.rNxy);
     P.ny = rfl(c.ny);
     P.nz = rfl(c.nz);
     P.nCells = rfl(c.nCells);
@@ -1119,9 +1151,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             int cell = L.ci;
             rec = cell < 0;
             if (rec) cell = ~cell;
-            L.ci = cell % P.nx;
-            L.cj = (cell / P.nx) % P.ny;
-            L.ck = cell / (P.nx * P.ny);
+            decode_cell(P, cell, L.ci, L.cj, L.ck);
             if (rec) {
                 L.a = L.vx;
                 L.b = L.vy;

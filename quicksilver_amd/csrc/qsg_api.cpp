@@ -122,6 +122,8 @@ void build(qsg_problem_s* p, int device) {
     d.ny = T.ny;
     d.nz = T.nz;
     d.nCells = T.nCells;
+    d.rNx = 1.0 / T.nx;
+    d.rNxy = 1.0 / ((double)T.nx * T.ny);
     d.dx = T.dx;
     d.dy = T.dy;
     d.dz = T.dz;

@@ -1730,12 +1730,19 @@ __global__ void __launch_bounds__(kFluxBlock) k_flux_partial(double* flux, size_
     if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
 }
 
-__global__ void k_flux_final(const double* partials, int n, double* result) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double acc = 0.0;
-        for (int q = 0; q < n; q++) acc += partials[q];
-        *result = acc;
+// Sum of the partials in a fixed order: strided per-thread sums, then the
+// block's tree (one thread summing 1024 partials from memory took ~50 us).
+__global__ void __launch_bounds__(kFluxBlock) k_flux_final(const double* partials, int n, double* result) {
+    __shared__ double sh[kFluxBlock];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += kFluxBlock) acc += partials[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *result = sh[0];
 }
 
 __global__ void k_clear(uint32_t* p, size_t words) {
@@ -2234,7 +2241,7 @@ void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partial
                    double* result) {
     hipStream_t s = (hipStream_t)ctx.stream;
     hipLaunchKernelGGL(k_flux_partial, dim3(nPartials), dim3(kFluxBlock), 0, s, flux, n, partials);
-    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(64), 0, s, partials, nPartials, result);
+    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(kFluxBlock), 0, s, partials, nPartials, result);
 }
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {

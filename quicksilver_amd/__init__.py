@@ -116,7 +116,7 @@ class Tuning(ctypes.Structure):
 
 
 # Launch shape used when none is set (the library's own default).
-DEFAULT_TUNING = {"block_size": 256, "waves_per_cu": 16, "refill_min": 16, "chunk": 64,
+DEFAULT_TUNING = {"block_size": 256, "waves_per_cu": 12, "refill_min": 1, "chunk": 32,
                   "stack_entries": 512, "sort_interval": 0, "waves_per_simd": 3,
                   "graphs": 1}
 

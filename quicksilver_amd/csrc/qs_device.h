@@ -133,7 +133,7 @@ constexpr uint32_t kStackCap = 512;
 // one collision per lane, at most 4 outputs each
 constexpr uint32_t kStackMargin = 256;
 inline size_t waveStackBytes(uint32_t nWaves, uint32_t cap) { return (size_t)nWaves * 12 * cap * 8; }
-constexpr TrackTuning kDefaultTuning = {256, 16, 16, 64, (int)kStackCap, 0, 3, 1};
+constexpr TrackTuning kDefaultTuning = {256, 12, 1, 32, (int)kStackCap, 0, 3, 1};
 
 // ---- batched form: many independent problems per launch ----
 // Per-problem mutable state of a batch (device memory).

@@ -239,7 +239,40 @@ void destroyProblem(qsg_problem_s* p) {
 }
 
 // cycleInit (main.cc:96-121): source, population control, low-weight roulette.
-void cycleInit(qsg_problem_s* p) {
+// Statistics of an enqueued cycle set-up, once its events have completed.
+void finishInitStats(qsg_problem_s* p) {
+    float ms = 0;
+    hipCheck(hipEventElapsedTime(&ms, p->evI0, p->evI1), "hipEventElapsedTime");
+    p->stats.init_seconds += ms * 1e-3;
+}
+
+// The flux sum and finalize timing of the previous cycle, once the stream
+// has passed them (the tracking synchronisation or the end of a run).
+void completeFinalize(qsg_problem_s* p) {
+    if (!p->finalizePending) return;
+    float ms = 0;
+    hipCheck(hipEventElapsedTime(&ms, p->evF0, p->evF1), "hipEventElapsedTime");
+    p->stats.finalize_seconds += ms * 1e-3;
+    if (p->fluxSumOut) *p->fluxSumOut = *p->hFluxSum;
+    p->fluxSumOut = nullptr;
+    p->finalizePending = false;
+}
+
+// After the first synchronisation of a cycle's tracking: the previous
+// cycle's finalize and this cycle's set-up have both completed.
+void completeSetUp(qsg_problem_s* p) {
+    completeFinalize(p);
+    if (p->popPending) {
+        p->popPending = false;
+        p->nPop = p->hpc->nPopOut;
+        finishInitStats(p);
+    }
+}
+
+// deferSync: leave the counters on the device; cycleTracking sizes its first
+// pass for the largest possible queue and reads them with its own
+// synchronisation (qsg_run), instead of a host round trip here.
+void cycleInit(qsg_problem_s* p, bool deferSync = false) {
     if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_init called twice without finalize");
     setDevice(p);
     const qs::ProblemTables& T = p->tables;
@@ -267,13 +300,16 @@ void cycleInit(qsg_problem_s* p) {
     if (p->queueSorted)
         qs::launchSort(c, p->dp, p->pop.v, &p->pc->nPopOut, proc.v, p->sortCounts, p->sortBlockSums);
     hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
+    p->phase = 1;
+    if (deferSync) {
+        p->popPending = true;
+        return;
+    }
     syncCounters(p);
+    completeFinalize(p);
     checkErrors(p);
     p->nPop = p->hpc->nPopOut;
-    float ms = 0;
-    hipCheck(hipEventElapsedTime(&ms, p->evI0, p->evI1), "hipEventElapsedTime");
-    p->stats.init_seconds += ms * 1e-3;
-    p->phase = 1;
+    finishInitStats(p);
 }
 
 // cycleTracking (main.cc:138-307): passes over the population until no
@@ -286,7 +322,10 @@ void cycleTracking(qsg_problem_s* p) {
     PBuf& census = p->proc[out];
     hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
     const uint32_t* nIn = &p->pc->nPopOut;
-    uint32_t nInHost = p->nPop;
+    // with the set-up's counters still on the device the first pass is sized
+    // for the largest queue; the kernel reads the real length
+    const bool firstUnknown = p->popPending;
+    uint32_t nInHost = firstUnknown ? p->pop.cap : p->nPop;
     qs::Particles in = p->queueSorted ? p->proc[p->cur].v : p->pop.v;
     int k = 0;
     while (nInHost > 0) {
@@ -307,25 +346,26 @@ void cycleTracking(qsg_problem_s* p) {
                             p->pc, p->dc);
         hipCheck(hipGetLastError(), "track pass launch");
         hipCheck(hipEventRecord(p->passEv[2 * k + 1], p->stream), "hipEventRecord");
-        uint32_t next = 0;
-        hipCheck(hipMemcpyAsync(&p->hpc->extraCount[k & 1], nOut, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                p->stream),
-                 "hipMemcpyAsync");
-        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-        next = p->hpc->extraCount[k & 1];
-        if (next > ex.cap) {
-            syncCounters(p);
-            checkErrors(p);
-            throw QsgError(QSG_ERR_CAPACITY, "secondary buffer overflow");
-        }
+        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");  // the last pass's record stands
+        // every counter (this pass's secondaries, census, errors, tallies)
+        // comes back with the one synchronisation a pass needs
+        syncCounters(p);
+        if (k == 0) completeSetUp(p);
+        checkErrors(p);
+        const uint32_t next = p->hpc->extraCount[k & 1];
+        if (next > ex.cap) throw QsgError(QSG_ERR_CAPACITY, "secondary buffer overflow");
         in = ex.v;
         nIn = nOut;
         nInHost = next;
         k++;
     }
-    hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
-    syncCounters(p);
-    checkErrors(p);
+    if (k == 0) {
+        // nothing to track: the counters are those of the set-up
+        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
+        syncCounters(p);
+        completeSetUp(p);
+        checkErrors(p);
+    }
     float ms = 0;
     hipCheck(hipEventElapsedTime(&ms, p->ev0, p->ev1), "hipEventElapsedTime");
     p->lastTrackingSeconds = ms * 1e-3;
@@ -342,7 +382,10 @@ void cycleTracking(qsg_problem_s* p) {
 }
 
 // cycleFinalize (main.cc:310-324) + Tallies::CycleFinalize (Tallies.cc:25-98)
-void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
+// deferSync: the flux sum is written to *fluxSum by the next
+// synchronisation (the next cycle's tracking, or completeFinalize at the end
+// of qsg_run); the balance counters are already on the host.
+void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum, bool deferSync = false) {
     if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
     setDevice(p);
     qs::LaunchCtx c = p->ctx();
@@ -353,13 +396,12 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
     hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
     hipCheck(hipMemcpyAsync(p->hFluxSum, p->fluxSumDev, sizeof(double), hipMemcpyDeviceToHost, p->stream),
              "hipMemcpyAsync");
-    syncCounters(p);
-    checkErrors(p);
+    // the tallies came back with the tracking's last synchronisation and
+    // no finalize kernel changes them
     const qs::DevCounters& d = *p->hdc;
+    p->finalizePending = true;
+    p->fluxSumOut = fluxSum;
     {
-        float ms = 0;
-        hipCheck(hipEventElapsedTime(&ms, p->evF0, p->evF1), "hipEventElapsedTime");
-        p->stats.finalize_seconds += ms * 1e-3;
         p->stats.segments += d.numSegments;
         p->stats.collisions += d.collision;
         p->stats.facet_crossings += d.numSegments - d.collision - d.census;
@@ -385,7 +427,10 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
     b.end = p->nCensus;
     qsgi::addBalance(p->cumulative, b);
     if (bal) *bal = b;
-    if (fluxSum) *fluxSum = *p->hFluxSum;
+    if (!deferSync) {
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        completeFinalize(p);
+    }
     p->startCount = p->nCensus;
     p->cur = p->censusBuf;
     p->cycle++;
@@ -631,11 +676,24 @@ int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum) {
 int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums, double* tracking_seconds) {
     QSG_GUARD({
         if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
-        for (int i = 0; i < n_cycles; i++) {
-            cycleInit(p);
-            cycleTracking(p);
-            if (tracking_seconds) tracking_seconds[i] = p->lastTrackingSeconds;
-            cycleFinalize(p, balances ? &balances[i] : nullptr, flux_sums ? &flux_sums[i] : nullptr);
+        // one host synchronisation per cycle (after each tracking pass): the
+        // set-up is not read back before tracking, and each flux sum is
+        // collected with the next cycle's synchronisation
+        try {
+            for (int i = 0; i < n_cycles; i++) {
+                cycleInit(p, true);
+                cycleTracking(p);
+                if (tracking_seconds) tracking_seconds[i] = p->lastTrackingSeconds;
+                cycleFinalize(p, balances ? &balances[i] : nullptr, flux_sums ? &flux_sums[i] : nullptr, true);
+            }
+            hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+            completeFinalize(p);
+        } catch (...) {
+            // nothing still in flight may write into the caller's arrays later
+            p->fluxSumOut = nullptr;
+            p->finalizePending = false;
+            p->popPending = false;
+            throw;
         }
     })
 }

@@ -88,6 +88,9 @@ struct qsg_problem_s {
     int cur = 0;               // proc[cur] holds this cycle's processing population
     uint64_t startCount = 0;   // census particles carried into this cycle
     uint32_t nPop = 0;         // processing population after population control
+    bool popPending = false;   // cycle set-up enqueued, its counters not read yet
+    double* fluxSumOut = nullptr;  // caller's slot for a flux sum still in flight
+    bool finalizePending = false;  // a finalize whose flux sum and timing are in flight
     uint32_t nCensus = 0;      // census count after tracking
     int censusBuf = -1;        // proc index holding the last census
     int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done

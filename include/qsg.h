@@ -109,7 +109,10 @@ int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum);
 
 /* Runs n cycles; per-cycle results into arrays of length n (may be NULL).
  * tracking_seconds (may be NULL) receives the device time of the tracking
- * phase of each cycle. */
+ * phase of each cycle.  The same results as n calls of the three phases,
+ * with one host synchronisation per cycle instead of four; the arrays are
+ * complete when the call returns (on an error, entries of the failing cycle
+ * may be unset). */
 int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums,
             double* tracking_seconds);
 

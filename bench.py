@@ -251,8 +251,9 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the problem is "
-              "sharded by particles or by whole problems across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one "
+              "GPU (across GPUs a problem would be split by particles, a batch by whole problems)",
+              file=sys.stderr)
         sys.exit(2)
 
     import quicksilver_amd as q

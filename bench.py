@@ -251,8 +251,9 @@ def main():
         copy_probe()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one "
-              "GPU (across GPUs a problem would be split by particles, a batch by whole problems)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library drives one "
+              "GPU per process (across GPUs a problem would be split by particles, a batch by whole "
+              "problems)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -122,3 +122,33 @@ def test_device_fp64_builtins_round_like_ieee():
         ulp = np.spacing(np.abs(h))
         assert np.max(np.abs(d - h) / ulp) <= 2.0
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["nested_small", "coral2_p2", "all_escape"])
+def test_run_matches_the_three_phase_calls(name):
+    """qsg_run synchronises once per cycle (the set-up's counters come back
+    with the first tracking pass, each flux sum with the next cycle); the
+    three per-cycle entry points synchronise after each phase.  Both must
+    give the same tallies bit for bit, the same stats counts, and flux sums
+    within the flux tolerance (the flux bins are FP64 atomic sums, whose
+    last bits depend on the order the device happens to add them in)."""
+    print(f"[parity] qsg_run vs cycle_init/tracking/finalize, deck={name}", flush=True)
+    g = load_golden(name)
+    n = len(g["cycles"])
+    with q.Problem.from_file(g["deck_path"]) as p:
+        fast = p.run(n)
+        st_fast = p.stats()
+    steps = []
+    with q.Problem.from_file(g["deck_path"]) as p:
+        for _ in range(n):
+            p.cycle_init()
+            p.cycle_tracking()
+            steps.append(p.cycle_finalize())
+        st_step = p.stats()
+    for c, ((b1, f1, _), (b2, f2)) in enumerate(zip(fast, steps)):
+        assert b1 == b2, f"cycle {c}: {b1} vs {b2}"
+        assert abs(f1 - f2) <= FLUX_RTOL * abs(f2), f"cycle {c} flux {f1!r} vs {f2!r}"
+    for k in ("segments", "collisions", "facet_crossings", "particle_loads", "cycles", "track_launches"):
+        assert st_fast[k] == st_step[k], k
+    _check_against(g, [(b, f, 0.0) for b, f, _ in fast], "qsg_run")

@@ -1692,20 +1692,95 @@ __device__ __forceinline__ void pop_emit(const DevProblem& P, const Particles& i
     }
 }
 
+// The next surviving output of one input particle, in pop_emit's order:
+// the splitFactor spawned copies, then the parent; under roulette an output
+// survives its own draw (its seed is the stream after that draw).  st is the
+// parent's stream, advanced by each spawn; j counts the outputs examined.
+__device__ __forceinline__ uint64_t pop_next_survivor(const PopDecision& d, double cutoff, uint64_t& st, int& j) {
+    for (;;) {
+        const bool copy = j < d.splitFactor;
+        uint64_t t = copy ? rng_spawn(st) : st;
+        j++;
+        if (!d.roulette) return t;
+        if (rng_sample(t) <= cutoff) return t;
+        if (!copy) return t;  // not reached: the parent is counted only if it survives
+    }
+}
+
+// Population control with the outputs written in order: one thread decides
+// for each input particle (pop_decide), a block scan gives each input its
+// output range, and the block writes its outputs a tile and a field at a
+// time through LDS, so every store instruction covers consecutive slots
+// (one thread writing its ~factor copies one after another touches factor
+// times more cache lines per store).  The block's output base is one atomic.
+constexpr uint32_t kPopTile = 1024;
+
 __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
                                                     Particles out, uint32_t outCap, int mode,
                                                     double factor, PassCounters* pc, DevCounters* dc) {
+    __shared__ uint32_t sWave[4];
+    __shared__ uint32_t sBase;
+    __shared__ double sBuf[kPopTile];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    PopDecision d = pop_decide(P, in, i, i < n, mode, factor);
-    uint32_t incl = wave_inclusive_scan(d.survivors);
-    uint32_t base = 0;
-    if (__lane_id() == 63 && incl) base = atomicAdd(&pc->nPopOut, incl);
-    base = __shfl(base, 63);
-    if (d.keep && d.survivors) pop_emit(P, in, i, d, out, base + incl - d.survivors, outCap, &pc->error);
-    uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
+    const uint32_t wid = threadIdx.x >> 6;
+    const PopDecision d = pop_decide(P, in, i, i < n, mode, factor);
+    const uint32_t surv = (d.keep && i < n) ? d.survivors : 0u;
+    const uint32_t incl = wave_inclusive_scan(surv);
+    if (__lane_id() == 63) sWave[wid] = incl;
+    __syncthreads();
+    uint32_t excl = incl - surv, total = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
+        if (w < wid) excl += sWave[w];
+        total += sWave[w];
+    }
+    if (threadIdx.x == 0) sBase = total ? atomicAdd(&pc->nPopOut, total) : 0u;
+    const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
     if (__lane_id() == 0) {
         if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
         if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
+    }
+    __syncthreads();
+    const uint32_t base = sBase;
+    if (total == 0) return;
+    const double cutoff = P.lowWeightCutoff;
+    double f[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t cell = 0;
+    if (surv) {
+#pragma unroll
+        for (int k = 0; k < 10; k++) f[k] = in.f(k)[i];
+        f[7] = d.roulette ? d.w / cutoff : d.w;  // weight after splitting and roulette
+        cell = in.cell()[i];
+    }
+    uint64_t st = d.seed;  // parent stream, advanced by the spawns
+    int j = 0;
+    for (uint32_t t0 = 0; t0 < total; t0 += kPopTile) {
+        const uint32_t t1 = min(t0 + kPopTile, total);
+        const uint32_t lo = max(excl, t0), hi = min(excl + surv, t1);
+        // fields 0-9 (the weight is field 7), then the seed, then the cell
+        for (int k = 0; k < 12; k++) {
+            if (k < 10) {
+                for (uint32_t o = lo; o < hi; o++) sBuf[o - t0] = f[k];
+            } else if (k == 10) {
+                for (uint32_t o = lo; o < hi; o++)
+                    sBuf[o - t0] = __longlong_as_double((long long)pop_next_survivor(d, cutoff, st, j));
+            } else {
+                for (uint32_t o = lo; o < hi; o++) reinterpret_cast<int32_t*>(sBuf)[o - t0] = cell;
+            }
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < t1 - t0; q += blockDim.x) {
+                const uint32_t o = base + t0 + q;
+                if (o >= outCap) {
+                    if (k == 0) atomicOr(&pc->error, 4u);
+                } else if (k < 10) {
+                    out.f(k)[o] = sBuf[q];
+                } else if (k == 10) {
+                    out.seed()[o] = (uint64_t)__double_as_longlong(sBuf[q]);
+                } else {
+                    out.cell()[o] = reinterpret_cast<const int32_t*>(sBuf)[q];
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 

@@ -1648,54 +1648,11 @@ __device__ __forceinline__ PopDecision pop_decide(const DevProblem& P, const Par
     return d;
 }
 
-__device__ __forceinline__ void pop_emit(const DevProblem& P, const Particles& in, uint32_t i, const PopDecision& d,
-                                         const Particles& out, uint32_t o, uint32_t outCap, uint32_t* error) {
-    const double cutoff = P.lowWeightCutoff;
-    const double x = in.x()[i], y = in.y()[i], z = in.z()[i], vx = in.vx()[i], vy = in.vy()[i], vz = in.vz()[i];
-    const double E = in.E()[i], tc = in.tc()[i], nmfp = in.nmfp()[i];
-    const int32_t cell = in.cell()[i];
-    const double wOut = d.roulette ? d.w / cutoff : d.w;
-    uint64_t s = d.seed;
-    auto emit = [&](uint64_t os) {
-        if (o < outCap) {
-            out.x()[o] = x;
-            out.y()[o] = y;
-            out.z()[o] = z;
-            out.vx()[o] = vx;
-            out.vy()[o] = vy;
-            out.vz()[o] = vz;
-            out.E()[o] = E;
-            out.w()[o] = wOut;
-            out.tc()[o] = tc;
-            out.nmfp()[o] = nmfp;
-            out.seed()[o] = os;
-            out.cell()[o] = cell;
-        } else {
-            atomicOr(error, 4u);
-        }
-        o++;
-    };
-    for (int j = 0; j < d.splitFactor; j++) {
-        uint64_t cs = rng_spawn(s);
-        if (d.roulette) {
-            uint64_t t = cs;
-            if (rng_sample(t) <= cutoff) emit(t);
-        } else {
-            emit(cs);
-        }
-    }
-    if (d.roulette) {
-        uint64_t t = s;
-        if (rng_sample(t) <= cutoff) emit(t);
-    } else {
-        emit(s);
-    }
-}
-
-// The next surviving output of one input particle, in pop_emit's order:
-// the splitFactor spawned copies, then the parent; under roulette an output
-// survives its own draw (its seed is the stream after that draw).  st is the
-// parent's stream, advanced by each spawn; j counts the outputs examined.
+// The next surviving output of one input particle.  The outputs are, in
+// order, the splitFactor spawned copies and then the parent; under roulette
+// an output survives its own draw (its seed is the stream after that draw).
+// st is the parent's stream, advanced by each spawn; j counts the outputs
+// examined.
 __device__ __forceinline__ uint64_t pop_next_survivor(const PopDecision& d, double cutoff, uint64_t& st, int& j) {
     for (;;) {
         const bool copy = j < d.splitFactor;
@@ -1712,19 +1669,20 @@ __device__ __forceinline__ uint64_t pop_next_survivor(const PopDecision& d, doub
 // output range, and the block writes its outputs a tile and a field at a
 // time through LDS, so every store instruction covers consecutive slots
 // (one thread writing its ~factor copies one after another touches factor
-// times more cache lines per store).  The block's output base is one atomic.
+// times more cache lines per store).  The block's output base is one atomic
+// on nPopOut.  Called by all 256 threads of a block, for 256 consecutive
+// inputs of one problem.
 constexpr uint32_t kPopTile = 1024;
 
-__global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
-                                                    Particles out, uint32_t outCap, int mode,
-                                                    double factor, PassCounters* pc, DevCounters* dc) {
+__device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
+                                          int mode, double factor, const Particles& out, uint32_t outCap,
+                                          uint32_t* nPopOut, uint32_t* error, DevCounters* dc) {
     __shared__ uint32_t sWave[4];
     __shared__ uint32_t sBase;
     __shared__ double sBuf[kPopTile];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t wid = threadIdx.x >> 6;
-    const PopDecision d = pop_decide(P, in, i, i < n, mode, factor);
-    const uint32_t surv = (d.keep && i < n) ? d.survivors : 0u;
+    const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
+    const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
     const uint32_t incl = wave_inclusive_scan(surv);
     if (__lane_id() == 63) sWave[wid] = incl;
     __syncthreads();
@@ -1733,7 +1691,7 @@ __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, 
         if (w < wid) excl += sWave[w];
         total += sWave[w];
     }
-    if (threadIdx.x == 0) sBase = total ? atomicAdd(&pc->nPopOut, total) : 0u;
+    if (threadIdx.x == 0) sBase = total ? atomicAdd(nPopOut, total) : 0u;
     const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
     if (__lane_id() == 0) {
         if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
@@ -1741,7 +1699,6 @@ __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, 
     }
     __syncthreads();
     const uint32_t base = sBase;
-    if (total == 0) return;
     const double cutoff = P.lowWeightCutoff;
     double f[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int32_t cell = 0;
@@ -1770,7 +1727,7 @@ __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, 
             for (uint32_t q = threadIdx.x; q < t1 - t0; q += blockDim.x) {
                 const uint32_t o = base + t0 + q;
                 if (o >= outCap) {
-                    if (k == 0) atomicOr(&pc->error, 4u);
+                    if (k == 0) atomicOr(error, 4u);
                 } else if (k < 10) {
                     out.f(k)[o] = sBuf[q];
                 } else if (k == 10) {
@@ -1782,6 +1739,13 @@ __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, 
             __syncthreads();
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
+                                                    Particles out, uint32_t outCap, int mode,
+                                                    double factor, PassCounters* pc, DevCounters* dc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    pop_block(P, in, i, i < n, mode, factor, out, outCap, &pc->nPopOut, &pc->error, dc);
 }
 
 constexpr int kFluxBlock = 256;
@@ -2019,7 +1983,7 @@ __global__ void k_batch_prep(BatchSlot* slots, int n, uint64_t* srcPrefix, uint6
             srcPrefix[p] = a;
             popPrefix[p] = b;
             if (slots[p].st->active && slots[p].st->error == 0) a += slots[p].sourcePerCycle;
-            b += slots[p].st->nProc;
+            b += ((uint64_t)slots[p].st->nProc + 255) / 256 * 256;  // whole blocks per problem
         }
         srcPrefix[n] = a;
         popPrefix[n] = b;
@@ -2048,40 +2012,18 @@ __global__ void __launch_bounds__(256) k_batch_source(BatchSlot* slots, int n, c
     }
 }
 
+// Population control of every problem of a batch: each problem's inputs
+// occupy whole 256-input blocks of the flat index space (popPrefix counts
+// them rounded up to 256), so one block serves one problem.
 __global__ void __launch_bounds__(256) k_batch_population(BatchSlot* slots, int n, const uint64_t* popPrefix) {
     const uint64_t total = popPrefix[n];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // all lanes of a wave run the same number of iterations
-    const uint64_t waveBase = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint64_t tw = waveBase; tw < total; tw += stride) {
-        const uint64_t t = tw + __lane_id();
-        const bool valid = t < total;
-        const int p = find_problem(popPrefix, n, valid ? t : total - 1);
+    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < total; b0 += (uint64_t)gridDim.x * blockDim.x) {
+        const int p = find_problem(popPrefix, n, b0);
         const BatchSlot& S = slots[p];
         BatchState* st = S.st;
-        const uint32_t i = (uint32_t)((valid ? t : total - 1) - popPrefix[p]);
-        PopDecision d = pop_decide(S.prob, S.proc[st->cycle & 1], i, valid, (int)st->mode, st->factor);
-        const int p0 = __shfl(p, 0);
-        const bool uniform = __ballot(p != p0) == 0;
-        uint32_t o;
-        if (uniform) {
-            uint32_t incl = wave_inclusive_scan(d.survivors);
-            uint32_t base = 0;
-            if (__lane_id() == 63 && incl) base = atomicAdd(&st->nPop, incl);
-            base = __shfl(base, 63);
-            o = base + incl - d.survivors;
-            uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
-            if (__lane_id() == 0) {
-                if (kills) atomicAdd(&S.dc->rr, (unsigned long long)kills);
-                if (splits) atomicAdd(&S.dc->split, (unsigned long long)splits);
-            }
-        } else {
-            o = d.survivors ? atomicAdd(&st->nPop, d.survivors) : 0u;
-            if (d.kills) atomicAdd(&S.dc->rr, (unsigned long long)d.kills);
-            if (d.splits) atomicAdd(&S.dc->split, (unsigned long long)d.splits);
-        }
-        if (d.keep && d.survivors)
-            pop_emit(S.prob, S.proc[st->cycle & 1], i, d, S.pop, o, S.cap, &st->error);
+        const uint32_t i = (uint32_t)(b0 - popPrefix[p]) + threadIdx.x;
+        pop_block(S.prob, S.proc[st->cycle & 1], i, i < st->nProc, (int)st->mode, st->factor, S.pop, S.cap,
+                  &st->nPop, &st->error, S.dc);
     }
 }
 

@@ -91,7 +91,8 @@ struct DevProblem {
 struct DevCounters {
     unsigned long long absorb, census, escape, collision, fission, produce, scatter, numSegments;
     unsigned long long source, rr, split;
-    unsigned long long pad[5];
+    double fluxSum;  // sum of this cycle's scalar-flux contributions (segment length x weight)
+    unsigned long long pad[4];
 };
 
 // Work-queue / append counters (device memory, cleared by the host).
@@ -201,9 +202,9 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
                      Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
                      PassCounters* pc, DevCounters* dc);
 
-// Sums the flux array into partials (deterministic order) and zeroes it.
-void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partials, int nPartials,
-                   double* result);
+// Zeroes the flux bins for the next cycle (the cycle's flux sum is
+// accumulated by the tracking kernel in DevCounters::fluxSum).
+void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n);
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
 // fluence[cell] += flux[cell][g] for g = 0..G-1 in order (Fluence::compute,
@@ -238,8 +239,7 @@ void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval
                      uint64_t* sortPrefix, uint32_t* counts, size_t totalKeys, uint32_t* blockSums);
 void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
                       const WaveStacks& ws);
-void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,
-                         double* fluxSums);
+void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, int partsPerProblem, double* fluxSums);
 
 // Device copies of single FP64 builtins, for rounding checks.
 // Nearest facet (with the nudge retry) of n rays: xyz, dir [n][3], cell [n]

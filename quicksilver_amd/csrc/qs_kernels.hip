@@ -1080,6 +1080,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     // wave-uniform (scalar) balance counters, updated from ballots at the
     // converged end of each iteration
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
+    double fluxAcc = 0.0;  // this lane's scalar-flux contributions
 
     for (;;) {
         // ---- refill idle lanes: the wave's stack first (most recent
@@ -1237,7 +1238,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             double t = seg / L.speed;
             L.tc -= t;
             if (L.tc < 0.0) L.tc = 0.0;
-            unsafeAtomicAdd(&P.flux[(size_t)cell * G + L.group], seg * L.w);
+            const double contrib = seg * L.w;
+            unsafeAtomicAdd(&P.flux[(size_t)cell * G + L.group], contrib);
+            fluxAcc += contrib;
         }
         ev |= kEvSegment;
         histSeg++;
@@ -1431,6 +1434,14 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                  __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
     }
     flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
+    // The cycle's flux sum (Tallies::ScalarFluxSum, Tallies.cc:146-171, the
+    // sum over all bins) as the sum of the contributions that filled the
+    // bins, so the bins need not be read back; only the summation order
+    // differs, as it already does between the device's atomic adds.
+    double fs = fluxAcc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fs += __shfl_xor(fs, o);
+    if (lane == 0 && fs != 0.0) unsafeAtomicAdd(&io.dc->fluxSum, fs);
 }
 
 // Tracks every particle of the wave's queue(s) to the end of its history
@@ -1750,38 +1761,17 @@ __global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, 
 
 constexpr int kFluxBlock = 256;
 
-__global__ void __launch_bounds__(kFluxBlock) k_flux_partial(double* flux, size_t n, double* partials) {
-    __shared__ double s[kFluxBlock];
-    const size_t per = (n + gridDim.x - 1) / gridDim.x;
-    const size_t lo = (size_t)blockIdx.x * per;
-    const size_t hi = lo + per < n ? lo + per : n;
-    double acc = 0.0;
-    for (size_t q = lo + threadIdx.x; q < hi; q += kFluxBlock) {
-        acc += flux[q];
-        flux[q] = 0.0;
-    }
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+// Tallies.cc:95: the flux bins restart from zero each cycle.  16-byte
+// stores over [lo, hi) of p (p 16-byte aligned, lo even).
+__device__ __forceinline__ void zero_doubles(double* p, size_t lo, size_t hi, size_t t, size_t stride) {
+    double2* q = reinterpret_cast<double2*>(p + lo);
+    const size_t n2 = (hi - lo) / 2;
+    for (size_t k = t; k < n2; k += stride) q[k] = make_double2(0.0, 0.0);
+    if (t == 0 && ((hi - lo) & 1)) p[hi - 1] = 0.0;
 }
 
-// Sum of the partials in a fixed order: strided per-thread sums, then the
-// block's tree (one thread summing 1024 partials from memory took ~50 us).
-__global__ void __launch_bounds__(kFluxBlock) k_flux_final(const double* partials, int n, double* result) {
-    __shared__ double sh[kFluxBlock];
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += kFluxBlock) acc += partials[q];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *result = sh[0];
+__global__ void __launch_bounds__(kFluxBlock) k_zero_flux(double* flux, size_t n) {
+    zero_doubles(flux, 0, n, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
 __global__ void k_clear(uint32_t* p, size_t words) {
@@ -2109,39 +2099,25 @@ __global__ void __launch_bounds__(256) k_batch_fluence(BatchSlot* slots) {
 
 // Flux sum per problem in a fixed order (Tallies.cc:146-171), flux reset
 // (Tallies.cc:95), and the cycle advance of every active problem.
-__global__ void __launch_bounds__(kFluxBlock) k_batch_flux_partial(BatchSlot* slots, int partsPerProblem,
-                                                                   double* partials) {
-    __shared__ double sh[kFluxBlock];
+__global__ void __launch_bounds__(kFluxBlock) k_batch_flux_zero(BatchSlot* slots, int partsPerProblem) {
     const int p = blockIdx.x / partsPerProblem, part = blockIdx.x % partsPerProblem;
     const BatchSlot& S = slots[p];
-    double acc = 0.0;
-    if (S.st->active) {
-        const size_t nf = (size_t)S.prob.nCells * S.prob.G;
-        const size_t per = (nf + partsPerProblem - 1) / partsPerProblem;
-        const size_t lo = (size_t)part * per;
-        const size_t hi = lo + per < nf ? lo + per : nf;
-        for (size_t q = lo + threadIdx.x; q < hi; q += kFluxBlock) {
-            acc += S.prob.flux[q];
-            S.prob.flux[q] = 0.0;
-        }
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = kFluxBlock / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+    if (!S.st->active) return;
+    const size_t nf = (size_t)S.prob.nCells * S.prob.G;
+    const size_t per = ((nf + partsPerProblem - 1) / partsPerProblem + 1) & ~(size_t)1;  // even slices
+    const size_t lo = (size_t)part * per;
+    if (lo >= nf) return;
+    const size_t hi = lo + per < nf ? lo + per : nf;
+    zero_doubles(S.prob.flux, lo, hi, threadIdx.x, blockDim.x);
 }
 
-__global__ void k_batch_flux_final(BatchSlot* slots, int n, int partsPerProblem, const double* partials,
-                                   double* fluxSums) {
+// Each active problem's flux sum (accumulated by the tracking kernel) and
+// its cycle advance.
+__global__ void k_batch_flux_final(BatchSlot* slots, int n, double* fluxSums) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         BatchState* st = slots[p].st;
         if (!st->active) continue;
-        double acc = 0.0;
-        for (int q = 0; q < partsPerProblem; q++) acc += partials[p * partsPerProblem + q];
-        fluxSums[p] = acc;
+        fluxSums[p] = slots[p].dc->fluxSum;
         st->cycle++;
     }
 }
@@ -2254,11 +2230,12 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
                        extra, nExtraOut, extraCap, pc, dc);
 }
 
-void launchFluxSum(const LaunchCtx& ctx, double* flux, size_t n, double* partials, int nPartials,
-                   double* result) {
-    hipStream_t s = (hipStream_t)ctx.stream;
-    hipLaunchKernelGGL(k_flux_partial, dim3(nPartials), dim3(kFluxBlock), 0, s, flux, n, partials);
-    hipLaunchKernelGGL(k_flux_final, dim3(1), dim3(kFluxBlock), 0, s, partials, nPartials, result);
+void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n) {
+    if (n == 0) return;
+    size_t blocks = (n / 2 + kFluxBlock - 1) / kFluxBlock;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_zero_flux, dim3((unsigned)blocks), dim3(kFluxBlock), 0, (hipStream_t)ctx.stream, flux, n);
 }
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {
@@ -2301,14 +2278,12 @@ void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKin
                        (hipStream_t)ctx.stream, slots, n, passKind, tune, ws);
 }
 
-void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, double* partials, int partsPerProblem,
-                         double* fluxSums) {
+void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, int partsPerProblem, double* fluxSums) {
     hipStream_t s = (hipStream_t)ctx.stream;
-    hipLaunchKernelGGL(k_batch_flux_partial, dim3((unsigned)(n * partsPerProblem)), dim3(kFluxBlock), 0, s, slots,
-                       partsPerProblem, partials);
+    hipLaunchKernelGGL(k_batch_flux_zero, dim3((unsigned)(n * partsPerProblem)), dim3(kFluxBlock), 0, s, slots,
+                       partsPerProblem);
     unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_batch_flux_final, dim3(blocks), dim3(256), 0, s, slots, n, partsPerProblem, partials,
-                       fluxSums);
+    hipLaunchKernelGGL(k_batch_flux_final, dim3(blocks), dim3(256), 0, s, slots, n, fluxSums);
 }
 
 namespace {

@@ -212,10 +212,6 @@ void build(qsg_problem_s* p, int device) {
                   "counter blocks must fit a pinned block");
     p->hpc = static_cast<qs::PassCounters*>(qsgi::acquirePinned(sizeof(qs::PassCounters)));
     p->hdc = static_cast<qs::DevCounters*>(qsgi::acquirePinned(sizeof(qs::DevCounters)));
-    p->hFluxSum = static_cast<double*>(qsgi::acquirePinned(sizeof(double)));
-    p->nPartials = 1024;
-    p->partials = p->dalloc<double>(p->nPartials);
-    p->fluxSumDev = p->dalloc<double>(1);
     // the uploads and clears above ran on the null stream, which the
     // problem's non-blocking stream does not wait for; other devices' and
     // other problems' work is not waited for
@@ -230,7 +226,6 @@ void destroyProblem(qsg_problem_s* p) {
     p->stacks.release();
     qsgi::releasePinned(p->hpc);
     qsgi::releasePinned(p->hdc);
-    qsgi::releasePinned(p->hFluxSum);
     for (hipEvent_t e : {p->ev0, p->ev1, p->evI0, p->evI1, p->evF0, p->evF1})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->passEv) (void)hipEventDestroy(e);
@@ -246,15 +241,13 @@ void finishInitStats(qsg_problem_s* p) {
     p->stats.init_seconds += ms * 1e-3;
 }
 
-// The flux sum and finalize timing of the previous cycle, once the stream
-// has passed them (the tracking synchronisation or the end of a run).
+// The finalize timing of the previous cycle, once the stream has passed it
+// (the tracking synchronisation or the end of a run).
 void completeFinalize(qsg_problem_s* p) {
     if (!p->finalizePending) return;
     float ms = 0;
     hipCheck(hipEventElapsedTime(&ms, p->evF0, p->evF1), "hipEventElapsedTime");
     p->stats.finalize_seconds += ms * 1e-3;
-    if (p->fluxSumOut) *p->fluxSumOut = *p->hFluxSum;
-    p->fluxSumOut = nullptr;
     p->finalizePending = false;
 }
 
@@ -382,9 +375,9 @@ void cycleTracking(qsg_problem_s* p) {
 }
 
 // cycleFinalize (main.cc:310-324) + Tallies::CycleFinalize (Tallies.cc:25-98)
-// deferSync: the flux sum is written to *fluxSum by the next
-// synchronisation (the next cycle's tracking, or completeFinalize at the end
-// of qsg_run); the balance counters are already on the host.
+// deferSync: the finalize kernels are not waited for here (the next
+// cycle's tracking synchronisation, or the end of qsg_run, collects their
+// timing); the balance counters and the flux sum are already on the host.
 void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum, bool deferSync = false) {
     if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
     setDevice(p);
@@ -392,15 +385,13 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum, bool def
     const qs::ProblemTables& T = p->tables;
     hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
     qs::launchFluence(c, p->dp);
-    qs::launchFluxSum(c, p->dp.flux, (size_t)T.nCells * T.nGroups, p->partials, p->nPartials, p->fluxSumDev);
+    qs::launchZeroFlux(c, p->dp.flux, (size_t)T.nCells * T.nGroups);
     hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
-    hipCheck(hipMemcpyAsync(p->hFluxSum, p->fluxSumDev, sizeof(double), hipMemcpyDeviceToHost, p->stream),
-             "hipMemcpyAsync");
-    // the tallies came back with the tracking's last synchronisation and
-    // no finalize kernel changes them
+    // the tallies and the flux sum came back with the tracking's last
+    // synchronisation, and no finalize kernel changes them
     const qs::DevCounters& d = *p->hdc;
     p->finalizePending = true;
-    p->fluxSumOut = fluxSum;
+    if (fluxSum) *fluxSum = d.fluxSum;
     {
         p->stats.segments += d.numSegments;
         p->stats.collisions += d.collision;
@@ -690,7 +681,6 @@ int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sum
             completeFinalize(p);
         } catch (...) {
             // nothing still in flight may write into the caller's arrays later
-            p->fluxSumOut = nullptr;
             p->finalizePending = false;
             p->popPending = false;
             throw;

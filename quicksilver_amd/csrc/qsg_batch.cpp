@@ -28,9 +28,8 @@ struct qsg_batch_s {
     size_t sortKeys = 0;
     size_t sortKeysCap = 0;
     bool graphsValid = false;  // cleared when memory the graphs hold is reallocated
-    double* dPartials = nullptr;
     double* dFluxSums = nullptr;
-    int partsPerProblem = 16;
+    int partsPerProblem = 64;  // blocks per problem for zeroing its flux bins
     qs::BatchState* hState = nullptr;  // pinned mirrors
     qs::DevCounters* hCounters = nullptr;
     double* hFluxSums = nullptr;
@@ -71,7 +70,7 @@ void destroyBatch(qsg_batch_s* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (void* d : {(void*)b->dSlots, (void*)b->dState, (void*)b->dCounters, (void*)b->dSrcPrefix,
-                    (void*)b->dPopPrefix, (void*)b->dPartials, (void*)b->dFluxSums, (void*)b->dKeyBase,
+                    (void*)b->dPopPrefix, (void*)b->dFluxSums, (void*)b->dKeyBase,
                     (void*)b->dSortPrefix, (void*)b->dSortCounts, (void*)b->dSortBlockSums})
         if (d) (void)hipFree(d);
     for (void* h : {(void*)b->hState, (void*)b->hCounters, (void*)b->hFluxSums})
@@ -188,7 +187,6 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, int capacity, int device,
         b->dCounters = devAlloc<qs::DevCounters>(c);
         b->dSrcPrefix = devAlloc<uint64_t>(c + 1);
         b->dPopPrefix = devAlloc<uint64_t>(c + 1);
-        b->dPartials = devAlloc<double>((size_t)c * b->partsPerProblem);
         b->dKeyBase = devAlloc<uint64_t>(c + 1);
         b->dSortPrefix = devAlloc<uint64_t>(c + 1);
         b->dFluxSums = devAlloc<double>(c);
@@ -262,7 +260,7 @@ void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& w
 void enqueueFinal(qsg_batch_s* b, const qs::LaunchCtx& c) {
     const int n = (int)b->probs.size();
     qs::launchBatchFluence(c, b->dSlots, n);
-    qs::launchBatchFinalize(c, b->dSlots, n, b->dPartials, b->partsPerProblem, b->dFluxSums);
+    qs::launchBatchFinalize(c, b->dSlots, n, b->partsPerProblem, b->dFluxSums);
     hipCheck(hipMemcpyAsync(b->hFluxSums, b->dFluxSums, n * sizeof(double), hipMemcpyDeviceToHost, b->stream),
              "hipMemcpyAsync");
     hipCheck(hipMemcpyAsync(b->hCounters, b->dCounters, n * sizeof(qs::DevCounters), hipMemcpyDeviceToHost,

@@ -89,8 +89,7 @@ struct qsg_problem_s {
     uint64_t startCount = 0;   // census particles carried into this cycle
     uint32_t nPop = 0;         // processing population after population control
     bool popPending = false;   // cycle set-up enqueued, its counters not read yet
-    double* fluxSumOut = nullptr;  // caller's slot for a flux sum still in flight
-    bool finalizePending = false;  // a finalize whose flux sum and timing are in flight
+    bool finalizePending = false;  // a finalize whose timing is still in flight
     uint32_t nCensus = 0;      // census count after tracking
     int censusBuf = -1;        // proc index holding the last census
     int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
@@ -104,10 +103,6 @@ struct qsg_problem_s {
     qs::DevCounters* dc = nullptr;
     qs::PassCounters* hpc = nullptr;  // pinned host mirrors
     qs::DevCounters* hdc = nullptr;
-    double* partials = nullptr;
-    int nPartials = 0;
-    double* fluxSumDev = nullptr;
-    double* hFluxSum = nullptr;
 
     qs::TrackTuning tune = qs::kDefaultTuning;
     qsgi::StackPool stacks;

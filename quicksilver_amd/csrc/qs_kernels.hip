@@ -6,11 +6,14 @@
 // cited next to it.  Integer RNG streams are bit-exact.  log/sin/cos come
 // from the device math library (within 1 ulp of glibc).
 //
-// Kernels
-//   k_source       one thread per source particle      MC_SourceNow.cc:68-151
-//   k_population   one thread per particle              PopulationControl.cc:71-171
-//   k_track        persistent waves over a work queue   CycleTracking.cc:15-119
-//   k_flux_*       deterministic flux sum + reset       Tallies.cc:95,146-171
+// Kernels (k_batch_* are the same over every problem of a batch)
+//   k_source       one thread per source particle         MC_SourceNow.cc:68-151
+//   k_population   one block per 256 inputs, ordered out   PopulationControl.cc:71-171
+//   k_track        persistent waves over a work queue      CycleTracking.cc:15-119
+//                  (also accumulates the cycle's flux sum, Tallies.cc:146-171)
+//   k_zero_flux    flux reset                              Tallies.cc:95
+//   k_fluence      CORAL2 fluence                          Tallies.cc:100-120
+//   k_sort_*       counting sort of a queue by (cell, energy band)
 #include <hip/hip_runtime.h>
 
 #include <mutex>

@@ -207,6 +207,8 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
 void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n);
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
+// Zeroes a problem's pass counters and balance counters in one launch.
+void launchClearCounters(const LaunchCtx& ctx, PassCounters* pc, DevCounters* dc);
 // fluence[cell] += flux[cell][g] for g = 0..G-1 in order (Fluence::compute,
 // Tallies.cc:100-120); call before the flux is reduced and zeroed.
 void launchFluence(const LaunchCtx& ctx, const DevProblem& P);

@@ -1777,6 +1777,14 @@ __global__ void __launch_bounds__(kFluxBlock) k_zero_flux(double* flux, size_t n
     zero_doubles(flux, 0, n, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
+// The cycle's counters, both blocks in one launch.
+__global__ void k_clear_counters(PassCounters* pc, DevCounters* dc) {
+    uint32_t* a = reinterpret_cast<uint32_t*>(pc);
+    uint32_t* b = reinterpret_cast<uint32_t*>(dc);
+    for (uint32_t q = threadIdx.x; q < sizeof(PassCounters) / 4; q += blockDim.x) a[q] = 0u;
+    for (uint32_t q = threadIdx.x; q < sizeof(DevCounters) / 4; q += blockDim.x) b[q] = 0u;
+}
+
 __global__ void k_clear(uint32_t* p, size_t words) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; q < words; q += (size_t)gridDim.x * blockDim.x) p[q] = 0u;
@@ -2239,6 +2247,10 @@ void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n) {
     if (blocks > 4096) blocks = 4096;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_zero_flux, dim3((unsigned)blocks), dim3(kFluxBlock), 0, (hipStream_t)ctx.stream, flux, n);
+}
+
+void launchClearCounters(const LaunchCtx& ctx, PassCounters* pc, DevCounters* dc) {
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, (hipStream_t)ctx.stream, pc, dc);
 }
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {

@@ -274,8 +274,7 @@ void cycleInit(qsg_problem_s* p, bool deferSync = false) {
     uint64_t n = p->startCount + T.sourcePerCycle;
     if (n > proc.cap) throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
     hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
-    qs::launchClear(c, p->dc, sizeof(qs::DevCounters));
-    qs::launchClear(c, p->pc, sizeof(qs::PassCounters));
+    qs::launchClearCounters(c, p->pc, p->dc);
     qs::launchSource(c, p->dp, proc.v, (uint32_t)p->startCount, T.sourcePerCycle, (uint32_t)p->cycle);
 
     // PopulationControl (PopulationControl.cc:20-65) on one rank
@@ -323,8 +322,12 @@ void cycleTracking(qsg_problem_s* p) {
     int k = 0;
     while (nInHost > 0) {
         uint32_t* nOut = &p->pc->extraCount[k & 1];
-        qs::launchClear(c, nOut, sizeof(uint32_t));
-        qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
+        if (k > 0) {
+            // the first pass's queue head and secondary count were cleared
+            // with the cycle's counters
+            qs::launchClear(c, nOut, sizeof(uint32_t));
+            qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
+        }
         PBuf& ex = p->extra[k & 1];
         while ((int)p->passEv.size() < 2 * (k + 1)) {
             hipEvent_t e;

@@ -106,6 +106,30 @@ struct PassCounters {
     uint32_t pad[2];
 };
 
+// Set-up values of a problem's current cycle, computed on the device from
+// the census count the previous cycle left there, so that a cycle can be
+// enqueued before the host has read that count.  While hold is nonzero every
+// kernel of the one-problem cycle returns at once: it is set when a cycle
+// needs more than its first tracking pass (a wave stack overflowed into the
+// secondary buffer), when a cycle reports an error, and when census + source
+// exceed the processing buffer; the host then finishes that cycle and
+// enqueues the next again.
+struct CycleCtl {
+    uint32_t hold;
+    uint32_t start;  // census particles carried into the cycle
+    uint32_t n;      // start + source (0 if that exceeds the buffer)
+    uint32_t mode;   // population control: 0 none, 1 roulette, 2 split
+    double factor;   // population-control factor
+    uint64_t pad;
+};
+
+// A problem's counters in one block, so that one copy brings them back.
+struct CycleBlock {
+    PassCounters pc;
+    DevCounters dc;
+    CycleCtl ctl;
+};
+
 struct TrackTuning {
     int blockSize;     // threads per workgroup (multiple of 64)
     int wavesPerCU;    // persistent waves per CU
@@ -175,15 +199,34 @@ struct LaunchCtx {
 
 size_t tableLdsBytes(const DevProblem& P);
 
-void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, uint32_t outBase,
-                  uint64_t nSource, uint32_t cycle);
+// Cycle set-up bookkeeping (main.cc:96-121, PopulationControl.cc:20-65):
+// `start` census particles carried in, the population-control factor and
+// mode into cb->ctl, the pass and balance counters cleared, hold cleared.
+void launchCycleSetup(const LaunchCtx& ctx, CycleBlock* cb, uint32_t start, uint32_t procCap, uint64_t nSource,
+                      uint64_t target, int loadBalance);
+// After a cycle's first pass (cycles enqueued ahead): sets hold if the pass
+// left secondaries in extraCount[0] or an error flag, copies cb to the pinned
+// host block `host`, and unless held or setupNext is 0 does the next
+// cycle's set-up from the census count the pass left.
+void launchCycleTail(const LaunchCtx& ctx, CycleBlock* cb, CycleBlock* host, int setupNext, uint32_t procCap,
+                     uint64_t nSource, uint64_t target, int loadBalance);
+
+// Source particles at out[ctl->start ...]; with flux non-null the same
+// launch also zeroes flux[0, nFlux) (the previous cycle's bins).
+void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, const CycleCtl* ctl,
+                  uint64_t nSource, uint32_t cycle, double* flux = nullptr, size_t nFlux = 0);
 
 // Population control (split / Russian roulette, PopulationControl.cc:20-122)
-// fused with low-weight roulette (PopulationControl.cc:127-171).
-// mode: 0 none, 1 roulette (factor < 1), 2 split (factor > 1).
-void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t n,
-                      Particles out, uint32_t outCap, int mode, double factor,
-                      PassCounters* pc, DevCounters* dc);
+// fused with low-weight roulette (PopulationControl.cc:127-171) over
+// in[0, cb->ctl.n) with ctl's mode and factor; inCap bounds ctl.n.  The
+// outputs go to out[0, cb->pc.nPopOut), the roulette and split counts to
+// cb->dc.  groups: scratch of popGroups(inCap) entries.
+struct PopGroup {
+    uint32_t total, kills, splits, base;
+};
+size_t popGroups(uint32_t inCap);
+void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t inCap,
+                      Particles out, uint32_t outCap, CycleBlock* cb, PopGroup* groups);
 
 // One tracking pass over `in` (count read from device memory *nIn; nInHost
 // only sizes the grid).  Census particles are appended to `census`
@@ -200,18 +243,16 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
                      Particles in, const uint32_t* nIn, uint32_t nInHost,
                      Particles census, uint32_t censusCap,
                      Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
-                     PassCounters* pc, DevCounters* dc);
+                     PassCounters* pc, DevCounters* dc, const uint32_t* hold);
 
 // Zeroes the flux bins for the next cycle (the cycle's flux sum is
 // accumulated by the tracking kernel in DevCounters::fluxSum).
-void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n);
+void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n, const uint32_t* hold);
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes);
-// Zeroes a problem's pass counters and balance counters in one launch.
-void launchClearCounters(const LaunchCtx& ctx, PassCounters* pc, DevCounters* dc);
 // fluence[cell] += flux[cell][g] for g = 0..G-1 in order (Fluence::compute,
-// Tallies.cc:100-120); call before the flux is reduced and zeroed.
-void launchFluence(const LaunchCtx& ctx, const DevProblem& P);
+// Tallies.cc:100-120); call before the flux is zeroed.
+void launchFluence(const LaunchCtx& ctx, const DevProblem& P, const uint32_t* hold);
 // The same for every active problem of a batch that accumulates fluence.
 void launchBatchFluence(const LaunchCtx& ctx, BatchSlot* slots, int n);
 
@@ -232,7 +273,7 @@ inline size_t scanBlocks(size_t m) { return (m + 1023) / 1024; }
 // Plain form: sorts in[0, *n) into out; counts has sortKeys(nCells)
 // entries, blockSums scanBlocks(keys).
 void launchSort(const LaunchCtx& ctx, const DevProblem& P, Particles in, const uint32_t* n, Particles out,
-                uint32_t* counts, uint32_t* blockSums);
+                uint32_t* counts, uint32_t* blockSums, const uint32_t* hold);
 // Batched form: every problem whose cycle is a multiple of `interval` sorts
 // its population-control output into proc[cycle & 1] and marks the state
 // sorted.  keyBase: [n + 1] exclusive prefix of the problems' key counts;

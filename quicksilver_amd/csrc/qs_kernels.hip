@@ -7,13 +7,21 @@
 // from the device math library (within 1 ulp of glibc).
 //
 // Kernels (k_batch_* are the same over every problem of a batch)
+//   k_cycle_setup  carried count, population factor, clears main.cc:96-121
 //   k_source       one thread per source particle         MC_SourceNow.cc:68-151
-//   k_population   one block per 256 inputs, ordered out   PopulationControl.cc:71-171
+//                  (optionally beside the previous cycle's flux reset)
+//   k_pop_count,   population control: group counts, their   PopulationControl.cc:71-171
+//   k_pop_scan,    scan into output bases, ordered writes
+//   k_population
 //   k_track        persistent waves over a work queue      CycleTracking.cc:15-119
 //                  (also accumulates the cycle's flux sum, Tallies.cc:146-171)
 //   k_zero_flux    flux reset                              Tallies.cc:95
 //   k_fluence      CORAL2 fluence                          Tallies.cc:100-120
 //   k_sort_*       counting sort of a queue by (cell, energy band)
+//   k_cycle_tail   end of a cycle's first pass: hold check, counters to the
+//                  host, next set-up (cycles enqueued ahead)
+// Every kernel of the one-problem cycle returns at once while CycleCtl::hold
+// is set (the host then finishes the held cycle itself).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -1473,7 +1481,9 @@ template <bool kLds, int kMinWaves>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves))) k_track(DevProblem P, TrackTuning tune, WaveStacks ws, Particles in,
                                                const uint32_t* __restrict__ nInPtr, Particles census,
                                                uint32_t censusCap, Particles extra, uint32_t* nExtraOut,
-                                               uint32_t extraCap, PassCounters* pc, DevCounters* dc) {
+                                               uint32_t extraCap, PassCounters* pc, DevCounters* dc,
+                                               const uint32_t* __restrict__ hold) {
+    if (*hold) return;
     extern __shared__ double lds[];
     __shared__ MatInfo sMat[kMaxMaterials];
     __shared__ int32_t sReact[kMaxMaterials * kMaxReactions];
@@ -1487,6 +1497,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
                   &pc->error, dc};
     SingleQueue acc = {P, T, io};
     run_queue(acc, tune, ws);
+}
+
+constexpr int kFluxBlock = 256;
+
+// Tallies.cc:95: the flux bins restart from zero each cycle.  16-byte
+// stores over [lo, hi) of p (p 16-byte aligned, lo even).
+__device__ __forceinline__ void zero_doubles(double* p, size_t lo, size_t hi, size_t t, size_t stride) {
+    double2* q = reinterpret_cast<double2*>(p + lo);
+    const size_t n2 = (hi - lo) / 2;
+    for (size_t k = t; k < n2; k += stride) q[k] = make_double2(0.0, 0.0);
+    if (t == 0 && ((hi - lo) & 1)) p[hi - 1] = 0.0;
 }
 
 // MC_SourceNow particle generation (MC_SourceNow.cc:68-151) with
@@ -1582,11 +1603,21 @@ __device__ __forceinline__ void source_particle(const DevProblem& P, const Parti
     out.cell()[o] = cell;
 }
 
-__global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, uint32_t outBase,
-                                                uint64_t nSource, uint32_t cycle) {
+// Blocks below srcBlocks generate source particles; the others (if any)
+// zero the previous cycle's flux bins, which the source does not touch: the
+// latency-bound source and the bandwidth-bound zeroing share the device.
+__global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, const CycleCtl* __restrict__ ctl,
+                                                uint64_t nSource, uint32_t cycle, uint32_t srcBlocks, double* flux,
+                                                size_t nFlux) {
+    if (ctl->hold) return;
+    if (blockIdx.x >= srcBlocks) {
+        zero_doubles(flux, 0, nFlux, (size_t)(blockIdx.x - srcBlocks) * blockDim.x + threadIdx.x,
+                     (size_t)(gridDim.x - srcBlocks) * blockDim.x);
+        return;
+    }
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nSource) return;
-    source_particle(P, out, outBase + (uint32_t)t, t, cycle);
+    source_particle(P, out, ctl->start + (uint32_t)t, t, cycle);
 }
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -1683,36 +1714,35 @@ __device__ __forceinline__ uint64_t pop_next_survivor(const PopDecision& d, doub
 // output range, and the block writes its outputs a tile and a field at a
 // time through LDS, so every store instruction covers consecutive slots
 // (one thread writing its ~factor copies one after another touches factor
-// times more cache lines per store).  The block's output base is one atomic
-// on nPopOut.  Called by all 256 threads of a block, for 256 consecutive
-// inputs of one problem.
+// times more cache lines per store).
 constexpr uint32_t kPopTile = 1024;
+constexpr uint32_t kPopMinGroup = 64;  // fewest inputs per block-wide population group
 
-__device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
-                                          int mode, double factor, const Particles& out, uint32_t outCap,
-                                          uint32_t* nPopOut, uint32_t* error, DevCounters* dc) {
-    __shared__ uint32_t sWave[4];
-    __shared__ uint32_t sBase;
-    __shared__ double sBuf[kPopTile];
+// Exclusive scan of v over the block (at most 16 waves); total receives the
+// block's sum.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t sWave[16];
     const uint32_t wid = threadIdx.x >> 6;
-    const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
-    const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
-    const uint32_t incl = wave_inclusive_scan(surv);
+    const uint32_t incl = wave_inclusive_scan(v);
     if (__lane_id() == 63) sWave[wid] = incl;
     __syncthreads();
-    uint32_t excl = incl - surv, total = 0;
+    uint32_t excl = incl - v;
+    total = 0;
     for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
         if (w < wid) excl += sWave[w];
         total += sWave[w];
     }
-    if (threadIdx.x == 0) sBase = total ? atomicAdd(nPopOut, total) : 0u;
-    const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
-    if (__lane_id() == 0) {
-        if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
-        if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
-    }
-    __syncthreads();
-    const uint32_t base = sBase;
+    __syncthreads();  // sWave may be written again after this
+    return excl;
+}
+
+// Writes the outputs of one group of inputs: thread t's surv outputs go to
+// out[base + excl ...]; total is the group's output count.  Slots at or
+// beyond outCap are not written (error bit 4).
+__device__ __forceinline__ void pop_write(const DevProblem& P, const Particles& in, uint32_t i, const PopDecision& d,
+                                          uint32_t surv, uint32_t excl, uint32_t total, uint32_t base,
+                                          const Particles& out, uint32_t outCap, uint32_t* error) {
+    __shared__ double sBuf[kPopTile];
     const double cutoff = P.lowWeightCutoff;
     double f[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int32_t cell = 0;
@@ -1755,34 +1785,217 @@ __device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& 
     }
 }
 
-__global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, uint32_t n,
-                                                    Particles out, uint32_t outCap, int mode,
-                                                    double factor, PassCounters* pc, DevCounters* dc) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    pop_block(P, in, i, i < n, mode, factor, out, outCap, &pc->nPopOut, &pc->error, dc);
+// One group of a batch problem's population control: the group's output
+// base is one atomic on nPopOut and the roulette and split counts one atomic
+// per wave.  Called by all threads of a block, thread t for input i
+// (valid: t holds an input; the inputs of a call are consecutive).
+__device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
+                                          int mode, double factor, const Particles& out, uint32_t outCap,
+                                          uint32_t* nPopOut, uint32_t* error, DevCounters* dc) {
+    __shared__ uint32_t sBase;
+    const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
+    const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
+    uint32_t total;
+    const uint32_t excl = block_excl_scan(surv, total);
+    if (threadIdx.x == 0) sBase = total ? atomicAdd(nPopOut, total) : 0u;
+    const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
+    if (__lane_id() == 0) {
+        if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
+        if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
+    }
+    __syncthreads();
+    pop_write(P, in, i, d, surv, excl, total, sBase, out, outCap, error);
 }
 
-constexpr int kFluxBlock = 256;
+// ---- one problem's population control: count, scan, write
+// Same-address atomics from every workgroup are slow on this device (they
+// serialise across the chiplets), so the one-problem form places each
+// group's outputs with a scan instead: k_pop_count stores each group's
+// output, roulette and split counts, k_pop_scan (one workgroup) turns them
+// into output bases and the balance counts, and k_population writes.  The
+// groups are the same in all three: group g holds inputs [g*size, (g+1)*size)
+// (pop_group_size), visited by block g mod gridDim.
 
-// Tallies.cc:95: the flux bins restart from zero each cycle.  16-byte
-// stores over [lo, hi) of p (p 16-byte aligned, lo even).
-__device__ __forceinline__ void zero_doubles(double* p, size_t lo, size_t hi, size_t t, size_t stride) {
-    double2* q = reinterpret_cast<double2*>(p + lo);
-    const size_t n2 = (hi - lo) / 2;
-    for (size_t k = t; k < n2; k += stride) q[k] = make_double2(0.0, 0.0);
-    if (t == 0 && ((hi - lo) & 1)) p[hi - 1] = 0.0;
+// Inputs per group: about 256 / factor when splitting (at least 64), so that
+// a group's outputs fill about one store round of the block; else 256.
+__device__ __forceinline__ uint32_t pop_group_size(const CycleCtl& c) {
+    if (c.mode != 2) return 256u;
+    const double g = 256.0 / ceil(c.factor);
+    return g < (double)kPopMinGroup ? kPopMinGroup : (uint32_t)g;
 }
 
-__global__ void __launch_bounds__(kFluxBlock) k_zero_flux(double* flux, size_t n) {
+__global__ void __launch_bounds__(256) k_pop_count(DevProblem P, Particles in, const CycleCtl* __restrict__ ctl,
+                                                   PopGroup* __restrict__ groups) {
+    __shared__ uint32_t sSum[3][4];
+    if (ctl->hold) return;
+    const uint32_t n = ctl->n, size = pop_group_size(*ctl);
+    const uint32_t nGroups = (n + size - 1) / size;
+    const int mode = (int)ctl->mode;
+    const double factor = ctl->factor;
+    const uint32_t wid = threadIdx.x >> 6;
+    for (uint32_t g = blockIdx.x; g < nGroups; g += gridDim.x) {
+        const uint32_t i = g * size + threadIdx.x;
+        const bool valid = threadIdx.x < size && i < n;
+        const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
+        const uint32_t surv = wave_sum((d.keep && valid) ? d.survivors : 0u);
+        const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
+        if (__lane_id() == 0) {
+            sSum[0][wid] = surv;
+            sSum[1][wid] = kills;
+            sSum[2][wid] = splits;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            PopGroup r = {0u, 0u, 0u, 0u};
+            for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
+                r.total += sSum[0][w];
+                r.kills += sSum[1][w];
+                r.splits += sSum[2][w];
+            }
+            groups[g] = r;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// One workgroup of 1024 threads: exclusive scan of the group output counts
+// into their bases, the roulette and split counts into the balance, the
+// output count into nPopOut.  More outputs than outCap: error bit 4, hold,
+// and nPopOut stays at outCap.
+__global__ void __launch_bounds__(1024) k_pop_scan(CycleBlock* cb, PopGroup* __restrict__ groups, uint32_t outCap) {
+    __shared__ unsigned long long sK[16], sS[16];
+    CycleCtl& c = cb->ctl;
+    if (c.hold) return;
+    const uint32_t size = pop_group_size(c);
+    const uint32_t nGroups = (c.n + size - 1) / size;
+    uint64_t carry = 0;
+    unsigned long long kills = 0, splits = 0;
+    for (uint32_t g0 = 0; g0 < nGroups; g0 += blockDim.x) {
+        const uint32_t g = g0 + threadIdx.x;
+        uint32_t v = 0;
+        if (g < nGroups) {
+            const PopGroup r = groups[g];
+            v = r.total;
+            kills += r.kills;
+            splits += r.splits;
+        }
+        uint32_t tot;
+        const uint32_t excl = block_excl_scan(v, tot);
+        if (g < nGroups) groups[g].base = (uint32_t)(carry + excl);
+        carry += tot;
+    }
+    kills = wave_sum64(kills);
+    splits = wave_sum64(splits);
+    const uint32_t wid = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        sK[wid] = kills;
+        sS[wid] = splits;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (uint32_t w = 1; w < (blockDim.x >> 6); w++) {
+        kills += sK[w];
+        splits += sS[w];
+    }
+    cb->dc.rr += kills;
+    cb->dc.split += splits;
+    cb->pc.nPopOut = carry < outCap ? (uint32_t)carry : outCap;
+    if (carry > outCap) {
+        cb->pc.error |= 4u;
+        c.hold = 1u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_population(DevProblem P, Particles in, Particles out, uint32_t outCap,
+                                                    PassCounters* pc, const CycleCtl* __restrict__ ctl,
+                                                    const PopGroup* __restrict__ groups) {
+    if (ctl->hold) return;
+    const uint32_t n = ctl->n, size = pop_group_size(*ctl);
+    const uint32_t nGroups = (n + size - 1) / size;
+    const int mode = (int)ctl->mode;
+    const double factor = ctl->factor;
+    for (uint32_t g = blockIdx.x; g < nGroups; g += gridDim.x) {
+        const uint32_t i = g * size + threadIdx.x;
+        const bool valid = threadIdx.x < size && i < n;
+        const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
+        const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
+        uint32_t total;
+        const uint32_t excl = block_excl_scan(surv, total);
+        pop_write(P, in, i, d, surv, excl, total, groups[g].base, out, outCap, &pc->error);
+    }
+}
+
+__global__ void __launch_bounds__(kFluxBlock) k_zero_flux(double* flux, size_t n, const uint32_t* __restrict__ hold) {
+    if (*hold) return;
     zero_doubles(flux, 0, n, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
-// The cycle's counters, both blocks in one launch.
-__global__ void k_clear_counters(PassCounters* pc, DevCounters* dc) {
-    uint32_t* a = reinterpret_cast<uint32_t*>(pc);
-    uint32_t* b = reinterpret_cast<uint32_t*>(dc);
+// Cycle set-up bookkeeping of one problem (main.cc:96-121 and
+// PopulationControl.cc:20-65, as the host would do it): the census count
+// carried in (start), the population-control factor and mode, the pass and
+// balance counters cleared, hold cleared.  Called by every thread of one
+// block, after any read of the counters.
+__device__ __forceinline__ void cycle_setup(CycleBlock* cb, uint32_t start, uint32_t procCap, uint64_t nSource,
+                                            uint64_t target, int loadBalance) {
+    uint32_t* a = reinterpret_cast<uint32_t*>(&cb->pc);
+    uint32_t* b = reinterpret_cast<uint32_t*>(&cb->dc);
     for (uint32_t q = threadIdx.x; q < sizeof(PassCounters) / 4; q += blockDim.x) a[q] = 0u;
     for (uint32_t q = threadIdx.x; q < sizeof(DevCounters) / 4; q += blockDim.x) b[q] = 0u;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint64_t n64 = (uint64_t)start + nSource;
+    const bool fits = n64 <= (uint64_t)procCap;
+    const uint32_t n = fits ? (uint32_t)n64 : 0u;
+    const double tgt = (double)target;
+    double factor;
+    if (loadBalance) factor = (n != 0) ? tgt / (double)n : 1.0;
+    else factor = tgt / (double)n;
+    uint32_t mode = 0;
+    if (factor != 1.0 && n != 0) mode = (factor < 1) ? 1u : ((factor > 1) ? 2u : 0u);
+    CycleCtl& c = cb->ctl;
+    c.start = start;
+    c.n = n;
+    c.mode = mode;
+    c.factor = factor;
+    if (!fits) cb->pc.error = 32u;
+    c.hold = fits ? 0u : 1u;
+}
+
+__global__ void k_cycle_setup(CycleBlock* cb, uint32_t start, uint32_t procCap, uint64_t nSource, uint64_t target,
+                              int loadBalance) {
+    cycle_setup(cb, start, procCap, nSource, target, loadBalance);
+}
+
+// The end of a cycle's first pass when cycles are enqueued ahead: hold is
+// set if the pass left secondaries or an error; every counter is copied to
+// the pinned host block `host`; then, unless held or setupNext is 0, the
+// next cycle's set-up from the census count the pass left.  One block.
+__global__ void k_cycle_tail(CycleBlock* cb, CycleBlock* host, int setupNext, uint32_t procCap, uint64_t nSource,
+                             uint64_t target, int loadBalance) {
+    __shared__ uint32_t sHold, sStart;
+    if (threadIdx.x == 0) {
+        uint32_t h = cb->ctl.hold;
+        if (!h && (cb->pc.extraCount[0] != 0 || cb->pc.error != 0)) {
+            h = 1u;
+            cb->ctl.hold = 1u;
+        }
+        sHold = h;
+        sStart = cb->pc.nCensus;
+    }
+    __syncthreads();
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(cb);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(host);
+    for (uint32_t q = threadIdx.x; q < sizeof(CycleBlock) / 4; q += blockDim.x) dst[q] = src[q];
+    __threadfence_system();
+    __syncthreads();  // the counters are read before they are cleared
+    if (sHold || !setupNext) return;
+    cycle_setup(cb, sStart, procCap, nSource, target, loadBalance);
 }
 
 __global__ void k_clear(uint32_t* p, size_t words) {
@@ -1834,7 +2047,8 @@ __global__ void __launch_bounds__(256) k_nearest_probe(DevProblem P, const doubl
 // Fluence::compute (Tallies.cc:100-120): per cell, the groups' flux added
 // one by one to the cumulative value.
 __global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux, double* __restrict__ fluence,
-                                                 int nCells, int G) {
+                                                 int nCells, int G, const uint32_t* __restrict__ hold) {
+    if (*hold) return;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nCells; c += gridDim.x * blockDim.x) {
         double v = fluence[c];
         const double* f = flux + (size_t)c * G;
@@ -1861,14 +2075,17 @@ __device__ __forceinline__ void copy_particle(const Particles& in, uint32_t i, c
 }
 
 __global__ void __launch_bounds__(256) k_sort_count(DevProblem P, Particles in, const uint32_t* __restrict__ nPtr,
-                                                    uint32_t* __restrict__ counts) {
+                                                    uint32_t* __restrict__ counts, const uint32_t* __restrict__ hold) {
+    if (*hold) return;
     const uint32_t n = *nPtr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         atomicAdd(&counts[sort_key(P, in, i)], 1u);
 }
 
 __global__ void __launch_bounds__(256) k_sort_scatter(DevProblem P, Particles in, const uint32_t* __restrict__ nPtr,
-                                                      uint32_t* __restrict__ offsets, Particles out) {
+                                                      uint32_t* __restrict__ offsets, Particles out,
+                                                      const uint32_t* __restrict__ hold) {
+    if (*hold) return;
     const uint32_t n = *nPtr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t o = atomicAdd(&offsets[sort_key(P, in, i)], 1u);
@@ -2185,23 +2402,45 @@ size_t tableLdsBytes(const DevProblem& P) {
     return sizeof(double) * ((size_t)(P.G + 1) + 2 * (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);
 }
 
-void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, uint32_t outBase,
-                  uint64_t nSource, uint32_t cycle) {
-    hipStream_t s = (hipStream_t)ctx.stream;
-    if (nSource == 0) return;
-    unsigned blocks = (unsigned)((nSource + 255) / 256);
-    hipLaunchKernelGGL(k_source, dim3(blocks), dim3(256), 0, s, P, out, outBase, nSource, cycle);
+void launchCycleSetup(const LaunchCtx& ctx, CycleBlock* cb, uint32_t start, uint32_t procCap, uint64_t nSource,
+                      uint64_t target, int loadBalance) {
+    hipLaunchKernelGGL(k_cycle_setup, dim3(1), dim3(64), 0, (hipStream_t)ctx.stream, cb, start, procCap, nSource,
+                       target, loadBalance);
 }
 
-void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t n,
-                      Particles out, uint32_t outCap, int mode, double factor, PassCounters* pc,
-                      DevCounters* dc) {
-    if (n == 0) return;
-    hipStream_t s = (hipStream_t)ctx.stream;
-    unsigned blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_population, dim3(blocks), dim3(256), 0, s, P, in, n, out, outCap, mode, factor,
-                       pc, dc);
+void launchCycleTail(const LaunchCtx& ctx, CycleBlock* cb, CycleBlock* host, int setupNext, uint32_t procCap,
+                     uint64_t nSource, uint64_t target, int loadBalance) {
+    hipLaunchKernelGGL(k_cycle_tail, dim3(1), dim3(64), 0, (hipStream_t)ctx.stream, cb, host, setupNext, procCap,
+                       nSource, target, loadBalance);
 }
+
+void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, const CycleCtl* ctl,
+                  uint64_t nSource, uint32_t cycle, double* flux, size_t nFlux) {
+    hipStream_t s = (hipStream_t)ctx.stream;
+    const unsigned srcBlocks = (unsigned)((nSource + 255) / 256);
+    unsigned zeroBlocks = 0;
+    if (flux && nFlux) {
+        size_t z = (nFlux / 2 + 255) / 256;
+        zeroBlocks = (unsigned)(z > 4096 ? 4096 : (z == 0 ? 1 : z));
+    }
+    if (srcBlocks + zeroBlocks == 0) return;
+    hipLaunchKernelGGL(k_source, dim3(srcBlocks + zeroBlocks), dim3(256), 0, s, P, out, ctl, nSource, cycle, srcBlocks,
+                       flux, nFlux);
+}
+
+void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t inCap,
+                      Particles out, uint32_t outCap, CycleBlock* cb, PopGroup* groups) {
+    if (inCap == 0) return;
+    hipStream_t s = (hipStream_t)ctx.stream;
+    unsigned blocks = (inCap + kPopMinGroup - 1) / kPopMinGroup;
+    const unsigned maxBlocks = (unsigned)ctx.numCUs * 8;
+    if (blocks > maxBlocks) blocks = maxBlocks;
+    hipLaunchKernelGGL(k_pop_count, dim3(blocks), dim3(256), 0, s, P, in, &cb->ctl, groups);
+    hipLaunchKernelGGL(k_pop_scan, dim3(1), dim3(1024), 0, s, cb, groups, outCap);
+    hipLaunchKernelGGL(k_population, dim3(blocks), dim3(256), 0, s, P, in, out, outCap, &cb->pc, &cb->ctl, groups);
+}
+
+size_t popGroups(uint32_t inCap) { return (inCap + kPopMinGroup - 1) / kPopMinGroup + 1; }
 
 constexpr size_t kLdsLimit = 62 * 1024;  // 64 KiB per workgroup less the static tables
 
@@ -2224,7 +2463,7 @@ long long batchTrackBlocks(const LaunchCtx& ctx, const TrackTuning& tune) {
 void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTuning& tune, const WaveStacks& ws,
                      Particles in, const uint32_t* nIn, uint32_t nInHost, Particles census,
                      uint32_t censusCap, Particles extra, uint32_t* nExtraOut, uint32_t extraCap,
-                     PassCounters* pc, DevCounters* dc) {
+                     PassCounters* pc, DevCounters* dc, const uint32_t* hold) {
     hipStream_t s = (hipStream_t)ctx.stream;
     const int block = tune.blockSize;
     const long long blocks = trackPassBlocks(ctx, tune, nInHost);
@@ -2238,19 +2477,16 @@ void launchTrackPass(const LaunchCtx& ctx, const DevProblem& P, const TrackTunin
         default: kern = inLds ? k_track<true, 3> : k_track<false, 3>; break;
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), lds, s, P, tune, ws, in, nIn, census, censusCap,
-                       extra, nExtraOut, extraCap, pc, dc);
+                       extra, nExtraOut, extraCap, pc, dc, hold);
 }
 
-void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n) {
+void launchZeroFlux(const LaunchCtx& ctx, double* flux, size_t n, const uint32_t* hold) {
     if (n == 0) return;
     size_t blocks = (n / 2 + kFluxBlock - 1) / kFluxBlock;
     if (blocks > 4096) blocks = 4096;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_zero_flux, dim3((unsigned)blocks), dim3(kFluxBlock), 0, (hipStream_t)ctx.stream, flux, n);
-}
-
-void launchClearCounters(const LaunchCtx& ctx, PassCounters* pc, DevCounters* dc) {
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, (hipStream_t)ctx.stream, pc, dc);
+    hipLaunchKernelGGL(k_zero_flux, dim3((unsigned)blocks), dim3(kFluxBlock), 0, (hipStream_t)ctx.stream, flux, n,
+                       hold);
 }
 
 void launchClear(const LaunchCtx& ctx, void* ptr, size_t bytes) {
@@ -2314,14 +2550,14 @@ void launchScan(hipStream_t s, uint32_t* data, size_t m, uint32_t* blockSums) {
 }  // namespace
 
 void launchSort(const LaunchCtx& ctx, const DevProblem& P, Particles in, const uint32_t* n, Particles out,
-                uint32_t* counts, uint32_t* blockSums) {
+                uint32_t* counts, uint32_t* blockSums, const uint32_t* hold) {
     hipStream_t s = (hipStream_t)ctx.stream;
     const size_t keys = sortKeys(P.nCells);
     launchClear(ctx, counts, keys * sizeof(uint32_t));
     const unsigned grid = (unsigned)ctx.numCUs * 8;
-    hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, s, P, in, n, counts);
+    hipLaunchKernelGGL(k_sort_count, dim3(grid), dim3(256), 0, s, P, in, n, counts, hold);
     launchScan(s, counts, keys, blockSums);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, P, in, n, counts, out);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(grid), dim3(256), 0, s, P, in, n, counts, out, hold);
 }
 
 void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval, const uint64_t* keyBase,
@@ -2339,12 +2575,12 @@ void launchBatchFluence(const LaunchCtx& ctx, BatchSlot* slots, int n) {
     hipLaunchKernelGGL(k_batch_fluence, dim3(64, (unsigned)n), dim3(256), 0, (hipStream_t)ctx.stream, slots);
 }
 
-void launchFluence(const LaunchCtx& ctx, const DevProblem& P) {
+void launchFluence(const LaunchCtx& ctx, const DevProblem& P, const uint32_t* hold) {
     if (!P.fluence) return;
     unsigned blocks = (unsigned)((P.nCells + 255) / 256);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fluence, dim3(blocks), dim3(256), 0, (hipStream_t)ctx.stream, P.flux, P.fluence, P.nCells,
-                       P.G);
+                       P.G, hold);
 }
 
 void launchNearestProbe(const LaunchCtx& ctx, const DevProblem& P, const double* xyz, const double* dir,

@@ -80,16 +80,19 @@ void copyParticles(const qs::Particles& d, const qsg_particles* h, uint64_t n, h
 
 void setDevice(qsg_problem_s* p) { hipCheck(hipSetDevice(p->device), "hipSetDevice"); }
 
+// Enqueues the copy of every counter into the pinned host block `slot`.
+void copyCounters(qsg_problem_s* p, int slot) {
+    hipCheck(hipMemcpyAsync(p->hcb[slot], p->cb, sizeof(qs::CycleBlock), hipMemcpyDeviceToHost, p->stream),
+             "hipMemcpyAsync");
+}
+
+// Every counter into hpc / hdc, after everything enqueued so far.
 void syncCounters(qsg_problem_s* p) {
-    hipCheck(hipMemcpyAsync(p->hpc, p->pc, sizeof(qs::PassCounters), hipMemcpyDeviceToHost, p->stream),
-             "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(p->hdc, p->dc, sizeof(qs::DevCounters), hipMemcpyDeviceToHost, p->stream),
-             "hipMemcpyAsync");
+    copyCounters(p, 0);
     hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
 }
 
-void checkErrors(qsg_problem_s* p) {
-    uint32_t e = p->hpc->error;
+void checkErrors(uint32_t e) {
     if (!e) return;
     std::string m = "device error flags:";
     if (e & 1u) m += " census buffer overflow;";
@@ -97,9 +100,11 @@ void checkErrors(qsg_problem_s* p) {
     if (e & 4u) m += " population-control buffer overflow;";
     if (e & 8u) m += " runaway particle history;";
     if (e & 16u) m += " fission produced more than 4 particles;";
-    throw QsgError((e & 7u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC,
+    if (e & 32u) m += " processing population exceeds capacity;";
+    throw QsgError((e & 39u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC,
                    m + " (raise QSG_CAPACITY_FACTOR for capacity errors)");
 }
+void checkErrors(qsg_problem_s* p) { checkErrors(p->hpc->error); }
 
 void build(qsg_problem_s* p, int device) {
     p->device = device;
@@ -204,14 +209,19 @@ void build(qsg_problem_s* p, int device) {
     p->extra[0] = makeParticles(p, cap);
     p->extra[1] = makeParticles(p, cap);
 
-    p->pc = p->dalloc<qs::PassCounters>(1);
-    p->dc = p->dalloc<qs::DevCounters>(1);
-    hipCheck(hipMemset(p->pc, 0, sizeof(qs::PassCounters)), "hipMemset");
-    hipCheck(hipMemset(p->dc, 0, sizeof(qs::DevCounters)), "hipMemset");
-    static_assert(sizeof(qs::PassCounters) <= qsgi::kPinnedBlock && sizeof(qs::DevCounters) <= qsgi::kPinnedBlock,
-                  "counter blocks must fit a pinned block");
-    p->hpc = static_cast<qs::PassCounters*>(qsgi::acquirePinned(sizeof(qs::PassCounters)));
-    p->hdc = static_cast<qs::DevCounters*>(qsgi::acquirePinned(sizeof(qs::DevCounters)));
+    p->popGroups = p->dalloc<qs::PopGroup>(qs::popGroups(cap));
+    p->cb = p->dalloc<qs::CycleBlock>(1);
+    hipCheck(hipMemset(p->cb, 0, sizeof(qs::CycleBlock)), "hipMemset");
+    p->pc = &p->cb->pc;
+    p->dc = &p->cb->dc;
+    p->ctl = &p->cb->ctl;
+    static_assert(sizeof(qs::CycleBlock) <= qsgi::kPinnedBlock, "the counter block must fit a pinned block");
+    for (qs::CycleBlock*& h : p->hcb) h = static_cast<qs::CycleBlock*>(qsgi::acquirePinned(sizeof(qs::CycleBlock)));
+    p->hpc = &p->hcb[0]->pc;
+    p->hdc = &p->hcb[0]->dc;
+    for (auto& r : p->runEv)
+        for (hipEvent_t* e : {&r.t0, &r.p1, &r.t1}) hipCheck(hipEventCreate(e), "hipEventCreate");
+    for (hipEvent_t* e : {&p->runStart, &p->runEnd}) hipCheck(hipEventCreate(e), "hipEventCreate");
     // the uploads and clears above ran on the null stream, which the
     // problem's non-blocking stream does not wait for; other devices' and
     // other problems' work is not waited for
@@ -224,187 +234,127 @@ void destroyProblem(qsg_problem_s* p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void* a : p->allocs) (void)hipFree(a);
     p->stacks.release();
-    qsgi::releasePinned(p->hpc);
-    qsgi::releasePinned(p->hdc);
+    for (qs::CycleBlock* h : p->hcb) qsgi::releasePinned(h);
     for (hipEvent_t e : {p->ev0, p->ev1, p->evI0, p->evI1, p->evF0, p->evF1})
+        if (e) (void)hipEventDestroy(e);
+    for (auto& r : p->runEv)
+        for (hipEvent_t e : {r.t0, r.p1, r.t1})
+            if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {p->runStart, p->runEnd})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->passEv) (void)hipEventDestroy(e);
     if (p->stream) qsgi::releaseStream(p->device, p->stream);
     delete p;
 }
 
-// cycleInit (main.cc:96-121): source, population control, low-weight roulette.
-// Statistics of an enqueued cycle set-up, once its events have completed.
-void finishInitStats(qsg_problem_s* p) {
+double elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
-    hipCheck(hipEventElapsedTime(&ms, p->evI0, p->evI1), "hipEventElapsedTime");
-    p->stats.init_seconds += ms * 1e-3;
+    hipCheck(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    return ms * 1e-3;
 }
 
-// The finalize timing of the previous cycle, once the stream has passed it
-// (the tracking synchronisation or the end of a run).
-void completeFinalize(qsg_problem_s* p) {
-    if (!p->finalizePending) return;
-    float ms = 0;
-    hipCheck(hipEventElapsedTime(&ms, p->evF0, p->evF1), "hipEventElapsedTime");
-    p->stats.finalize_seconds += ms * 1e-3;
-    p->finalizePending = false;
-}
-
-// After the first synchronisation of a cycle's tracking: the previous
-// cycle's finalize and this cycle's set-up have both completed.
-void completeSetUp(qsg_problem_s* p) {
-    completeFinalize(p);
-    if (p->popPending) {
-        p->popPending = false;
-        p->nPop = p->hpc->nPopOut;
-        finishInitStats(p);
-    }
-}
-
-// deferSync: leave the counters on the device; cycleTracking sizes its first
-// pass for the largest possible queue and reads them with its own
-// synchronisation (qsg_run), instead of a host round trip here.
-void cycleInit(qsg_problem_s* p, bool deferSync = false) {
-    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_init called twice without finalize");
-    setDevice(p);
+// Cycle set-up (cycleInit, main.cc:96-121) of cycle number `cycle` with
+// processing buffer proc[cur]: the bookkeeping (with setUp, for p->startCount
+// census particles carried in; else a cycle tail has done it), source,
+// population control and low-weight roulette; every sort_interval cycles the
+// queue is then sorted by (cell, energy band) into proc[cur], whose
+// particles the population control has already consumed.  zeroFlux: the
+// source launch also zeroes the previous cycle's flux bins.  Returns whether
+// the queue was sorted (the first pass then reads proc[cur], else pop).
+bool enqueueSetUp(qsg_problem_s* p, int cycle, int cur, bool setUp, bool zeroFlux) {
     const qs::ProblemTables& T = p->tables;
-    qs::LaunchCtx c = p->ctx();
-    PBuf& proc = p->proc[p->cur];
-    uint64_t n = p->startCount + T.sourcePerCycle;
-    if (n > proc.cap) throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
-    hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
-    qs::launchClearCounters(c, p->pc, p->dc);
-    qs::launchSource(c, p->dp, proc.v, (uint32_t)p->startCount, T.sourcePerCycle, (uint32_t)p->cycle);
-
+    const qs::LaunchCtx c = p->ctx();
+    PBuf& proc = p->proc[cur];
+    if (setUp)
+        qs::launchCycleSetup(c, p->cb, (uint32_t)p->startCount, proc.cap, T.sourcePerCycle, T.nParticles,
+                             T.loadBalance);
+    qs::launchSource(c, p->dp, proc.v, p->ctl, T.sourcePerCycle, (uint32_t)cycle, zeroFlux ? p->dp.flux : nullptr,
+                     zeroFlux ? (size_t)T.nCells * T.nGroups : 0);
     // PopulationControl (PopulationControl.cc:20-65) on one rank
-    double target = (double)T.nParticles;
-    double factor;
-    if (T.loadBalance) factor = (n != 0) ? target / (double)n : 1.0;
-    else factor = target / (double)n;
-    int mode = 0;
-    if (factor != 1.0 && n != 0) mode = (factor < 1) ? 1 : ((factor > 1) ? 2 : 0);
-    qs::launchPopulation(c, p->dp, proc.v, (uint32_t)n, p->pop.v, p->pop.cap, mode, factor, p->pc, p->dc);
-    // every sort_interval cycles the queue is sorted by (cell, energy band)
-    // into this cycle's processing buffer, whose particles the population
-    // control has already consumed
-    p->queueSorted = p->tune.sortInterval > 0 && (p->cycle % p->tune.sortInterval) == 0;
-    if (p->queueSorted)
-        qs::launchSort(c, p->dp, p->pop.v, &p->pc->nPopOut, proc.v, p->sortCounts, p->sortBlockSums);
-    hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
-    p->phase = 1;
-    if (deferSync) {
-        p->popPending = true;
-        return;
-    }
-    syncCounters(p);
-    completeFinalize(p);
-    checkErrors(p);
-    p->nPop = p->hpc->nPopOut;
-    finishInitStats(p);
+    qs::launchPopulation(c, p->dp, proc.v, proc.cap, p->pop.v, p->pop.cap, p->cb, p->popGroups);
+    const bool sorted = p->tune.sortInterval > 0 && (cycle % p->tune.sortInterval) == 0;
+    if (sorted)
+        qs::launchSort(c, p->dp, p->pop.v, &p->pc->nPopOut, proc.v, p->sortCounts, p->sortBlockSums, &p->ctl->hold);
+    return sorted;
 }
 
-// cycleTracking (main.cc:138-307): passes over the population until no
-// secondaries remain.
-void cycleTracking(qsg_problem_s* p) {
-    if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "cycle_tracking called before cycle_init");
-    setDevice(p);
-    qs::LaunchCtx c = p->ctx();
-    const int out = 1 - p->cur;  // census goes to the other processing buffer
-    PBuf& census = p->proc[out];
-    hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
-    const uint32_t* nIn = &p->pc->nPopOut;
-    // with the set-up's counters still on the device the first pass is sized
-    // for the largest queue; the kernel reads the real length
-    const bool firstUnknown = p->popPending;
-    uint32_t nInHost = firstUnknown ? p->pop.cap : p->nPop;
-    qs::Particles in = p->queueSorted ? p->proc[p->cur].v : p->pop.v;
-    int k = 0;
+// Tracking pass k of the cycle whose processing buffer is proc[cur], over
+// `in` (length *nIn on the device, at most nInHost): census particles go to
+// proc[1 - cur], secondaries that do not fit a wave stack to extra[k & 1].
+// Pass k > 0 first clears its queue head and secondary count (pass 0 uses
+// those the set-up cleared).
+void enqueuePass(qsg_problem_s* p, int k, int cur, qs::Particles in, const uint32_t* nIn, uint32_t nInHost) {
+    const qs::LaunchCtx c = p->ctx();
+    uint32_t* nOut = &p->pc->extraCount[k & 1];
+    if (k > 0) {
+        qs::launchClear(c, nOut, sizeof(uint32_t));
+        qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
+    }
+    PBuf& census = p->proc[1 - cur];
+    PBuf& ex = p->extra[k & 1];
+    const long long blocks = qs::trackPassBlocks(c, p->tune, nInHost);
+    const qs::WaveStacks ws =
+        p->stacks.ensure((uint32_t)(blocks * (p->tune.blockSize / 64)), (uint32_t)p->tune.stackCap, p->stream);
+    qs::launchTrackPass(c, p->dp, p->tune, ws, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap, p->pc,
+                        p->dc, &p->ctl->hold);
+    hipCheck(hipGetLastError(), "track pass launch");
+}
+
+// Passes k, k + 1, ... (cycleTracking, main.cc:138-307), the first over
+// `in`, each later one over the secondaries its predecessor left, until a
+// pass leaves none.  One synchronisation per pass brings every counter back
+// into hpc / hdc; `end` is recorded after each pass.  Returns the number of
+// passes run.
+int runPasses(qsg_problem_s* p, int k, int cur, qs::Particles in, const uint32_t* nIn, uint32_t nInHost,
+              hipEvent_t end) {
+    int q = 0;
     while (nInHost > 0) {
-        uint32_t* nOut = &p->pc->extraCount[k & 1];
-        if (k > 0) {
-            // the first pass's queue head and secondary count were cleared
-            // with the cycle's counters
-            qs::launchClear(c, nOut, sizeof(uint32_t));
-            qs::launchClear(c, &p->pc->head, sizeof(uint32_t));
-        }
-        PBuf& ex = p->extra[k & 1];
-        while ((int)p->passEv.size() < 2 * (k + 1)) {
+        while ((int)p->passEv.size() < 2 * (q + 1)) {
             hipEvent_t e;
             hipCheck(hipEventCreate(&e), "hipEventCreate");
             p->passEv.push_back(e);
         }
-        hipCheck(hipEventRecord(p->passEv[2 * k], p->stream), "hipEventRecord");
-        const long long blocks = qs::trackPassBlocks(c, p->tune, nInHost);
-        const qs::WaveStacks ws =
-            p->stacks.ensure((uint32_t)(blocks * (p->tune.blockSize / 64)), (uint32_t)p->tune.stackCap, p->stream);
-        qs::launchTrackPass(c, p->dp, p->tune, ws, in, nIn, nInHost, census.v, census.cap, ex.v, nOut, ex.cap,
-                            p->pc, p->dc);
-        hipCheck(hipGetLastError(), "track pass launch");
-        hipCheck(hipEventRecord(p->passEv[2 * k + 1], p->stream), "hipEventRecord");
-        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");  // the last pass's record stands
-        // every counter (this pass's secondaries, census, errors, tallies)
-        // comes back with the one synchronisation a pass needs
+        hipCheck(hipEventRecord(p->passEv[2 * q], p->stream), "hipEventRecord");
+        enqueuePass(p, k, cur, in, nIn, nInHost);
+        hipCheck(hipEventRecord(p->passEv[2 * q + 1], p->stream), "hipEventRecord");
+        hipCheck(hipEventRecord(end, p->stream), "hipEventRecord");  // the last pass's record stands
         syncCounters(p);
-        if (k == 0) completeSetUp(p);
         checkErrors(p);
         const uint32_t next = p->hpc->extraCount[k & 1];
-        if (next > ex.cap) throw QsgError(QSG_ERR_CAPACITY, "secondary buffer overflow");
-        in = ex.v;
-        nIn = nOut;
+        if (next > p->extra[k & 1].cap) throw QsgError(QSG_ERR_CAPACITY, "secondary buffer overflow");
+        in = p->extra[k & 1].v;
+        nIn = &p->pc->extraCount[k & 1];
         nInHost = next;
         k++;
+        q++;
     }
-    if (k == 0) {
-        // nothing to track: the counters are those of the set-up
-        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
-        syncCounters(p);
-        completeSetUp(p);
-        checkErrors(p);
-    }
-    float ms = 0;
-    hipCheck(hipEventElapsedTime(&ms, p->ev0, p->ev1), "hipEventElapsedTime");
-    p->lastTrackingSeconds = ms * 1e-3;
-    p->stats.tracking_seconds += ms * 1e-3;
-    for (int q = 0; q < k; q++) {
-        float pm = 0;
-        hipCheck(hipEventElapsedTime(&pm, p->passEv[2 * q], p->passEv[2 * q + 1]), "hipEventElapsedTime");
-        p->stats.track_kernel_seconds += pm * 1e-3;
-    }
-    p->stats.track_launches += (uint64_t)k;
-    p->nCensus = p->hpc->nCensus;
-    p->censusBuf = out;
-    p->phase = 2;
+    for (int i = 0; i < q; i++) p->stats.track_kernel_seconds += elapsed(p->passEv[2 * i], p->passEv[2 * i + 1]);
+    p->stats.track_launches += (uint64_t)q;
+    return q;
 }
 
-// cycleFinalize (main.cc:310-324) + Tallies::CycleFinalize (Tallies.cc:25-98)
-// deferSync: the finalize kernels are not waited for here (the next
-// cycle's tracking synchronisation, or the end of qsg_run, collects their
-// timing); the balance counters and the flux sum are already on the host.
-void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum, bool deferSync = false) {
-    if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
-    setDevice(p);
-    qs::LaunchCtx c = p->ctx();
+// Fluence, then the flux bins restart from zero (Tallies::CycleFinalize,
+// Tallies.cc:25-98).  The cycle's flux sum was accumulated by the passes.
+void enqueueFinalize(qsg_problem_s* p) {
+    const qs::LaunchCtx c = p->ctx();
+    qs::launchFluence(c, p->dp, &p->ctl->hold);
+    qs::launchZeroFlux(c, p->dp.flux, (size_t)p->tables.nCells * p->tables.nGroups, &p->ctl->hold);
+}
+
+// The balance of a cycle from its final counters (cycleFinalize,
+// main.cc:310-324); the problem then stands at the next cycle boundary with
+// the census as its processing population.
+void closeCycle(qsg_problem_s* p, const qs::DevCounters& d, uint32_t nCensus, qsg_balance* bal, double* fluxSum) {
     const qs::ProblemTables& T = p->tables;
-    hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
-    qs::launchFluence(c, p->dp);
-    qs::launchZeroFlux(c, p->dp.flux, (size_t)T.nCells * T.nGroups);
-    hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
-    // the tallies and the flux sum came back with the tracking's last
-    // synchronisation, and no finalize kernel changes them
-    const qs::DevCounters& d = *p->hdc;
-    p->finalizePending = true;
     if (fluxSum) *fluxSum = d.fluxSum;
-    {
-        p->stats.segments += d.numSegments;
-        p->stats.collisions += d.collision;
-        p->stats.facet_crossings += d.numSegments - d.collision - d.census;
-        // particle loads: every population-control survivor plus every
-        // secondary record (start + source + split - rr + produce when each
-        // fission emits >= 2 particles)
-        p->stats.particle_loads += p->startCount + T.sourcePerCycle + d.split - d.rr + d.produce;
-        p->stats.cycles += 1;
-    }
+    p->stats.segments += d.numSegments;
+    p->stats.collisions += d.collision;
+    p->stats.facet_crossings += d.numSegments - d.collision - d.census;
+    // particle loads: every population-control survivor plus every
+    // secondary record (start + source + split - rr + produce when each
+    // fission emits >= 2 particles)
+    p->stats.particle_loads += p->startCount + T.sourcePerCycle + d.split - d.rr + d.produce;
+    p->stats.cycles += 1;
     qsg_balance b;
     b.start = p->startCount;
     b.source = T.sourcePerCycle;
@@ -418,17 +368,157 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum, bool def
     b.escape = d.escape;
     b.census = d.census;
     b.num_segments = d.numSegments;
-    b.end = p->nCensus;
+    b.end = nCensus;
     qsgi::addBalance(p->cumulative, b);
     if (bal) *bal = b;
-    if (!deferSync) {
-        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-        completeFinalize(p);
-    }
-    p->startCount = p->nCensus;
+    p->nCensus = nCensus;
+    p->censusBuf = 1 - p->cur;
+    p->startCount = nCensus;
     p->cur = p->censusBuf;
     p->cycle++;
     p->phase = 0;
+}
+
+// ---- the three per-cycle entry points: each synchronises once it is done
+
+void cycleInit(qsg_problem_s* p) {
+    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_init called twice without finalize");
+    setDevice(p);
+    if (p->startCount + p->tables.sourcePerCycle > p->proc[p->cur].cap)
+        throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
+    hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
+    p->queueSorted = enqueueSetUp(p, p->cycle, p->cur, true, false);
+    hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
+    syncCounters(p);
+    checkErrors(p);
+    p->nPop = p->hpc->nPopOut;
+    p->stats.init_seconds += elapsed(p->evI0, p->evI1);
+    p->phase = 1;
+}
+
+void cycleTracking(qsg_problem_s* p) {
+    if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "cycle_tracking called before cycle_init");
+    setDevice(p);
+    const int cur = p->cur;
+    hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
+    const int passes = runPasses(p, 0, cur, p->queueSorted ? p->proc[cur].v : p->pop.v, &p->pc->nPopOut, p->nPop,
+                                 p->ev1);
+    if (passes == 0) {
+        // nothing to track: the counters are those the set-up left
+        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
+        hipCheck(hipEventSynchronize(p->ev1), "hipEventSynchronize");
+    }
+    p->lastTrackingSeconds = elapsed(p->ev0, p->ev1);
+    p->stats.tracking_seconds += p->lastTrackingSeconds;
+    p->nCensus = p->hpc->nCensus;
+    p->censusBuf = 1 - cur;
+    p->phase = 2;
+}
+
+void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
+    if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
+    setDevice(p);
+    hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
+    enqueueFinalize(p);
+    hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
+    hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    p->stats.finalize_seconds += elapsed(p->evF0, p->evF1);
+    // the tallies and the flux sum came back with the tracking's last
+    // synchronisation, and no finalize kernel changes them
+    closeCycle(p, *p->hdc, p->nCensus, bal, fluxSum);
+}
+
+// ---- qsg_run: cycles enqueued ahead of the host
+//
+// Cycle i + 1 is enqueued before the host waits for cycle i, so the device
+// goes from one cycle to the next without waiting on the host.  After a
+// cycle's first pass one small kernel (k_cycle_tail) copies the counters to
+// the host and does the next cycle's set-up bookkeeping from the census
+// count on the device; the next cycle's source launch also zeroes the flux
+// bins.  A cycle whose first pass leaves secondaries (a wave stack
+// overflowed) or an error sets CycleCtl::hold, and every kernel enqueued
+// after it returns at once; the host then runs that cycle's further passes
+// itself and enqueues the next cycle again.  Timing: events before and after
+// each first pass only; the device time between two first passes (copy,
+// finalize, set-up) is counted as set-up, the last cycle's as finalize.
+void runCycles(qsg_problem_s* p, int nCycles, qsg_balance* bals, double* fluxSums, double* trackSec) {
+    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "qsg_run needs a problem at a cycle boundary");
+    if (nCycles <= 0) return;
+    setDevice(p);
+    const qs::ProblemTables& T = p->tables;
+    if (p->startCount + T.sourcePerCycle > p->proc[p->cur].cap)
+        throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
+    const qs::LaunchCtx c = p->ctx();
+    const int cycle0 = p->cycle, cur0 = p->cur;
+    // cycle i: (the previous cycle's fluence,) set-up, first pass, tail;
+    // setUp: the bookkeeping too (no tail has done it)
+    auto enqueueCycle = [&](int i, bool setUp) {
+        auto& e = p->runEv[i & 3];
+        const int cur = cur0 ^ (i & 1);
+        if (i > 0) qs::launchFluence(c, p->dp, &p->ctl->hold);
+        const bool sorted = enqueueSetUp(p, cycle0 + i, cur, setUp, i > 0);
+        hipCheck(hipEventRecord(e.t0, p->stream), "hipEventRecord");
+        enqueuePass(p, 0, cur, sorted ? p->proc[cur].v : p->pop.v, &p->pc->nPopOut, p->pop.cap);
+        hipCheck(hipEventRecord(e.p1, p->stream), "hipEventRecord");
+        qs::launchCycleTail(c, p->cb, p->hcb[i & 1], i + 1 < nCycles ? 1 : 0, p->proc[cur ^ 1].cap,
+                            T.sourcePerCycle, T.nParticles, T.loadBalance);
+    };
+    // everything after cycle i's tail up to the next cycle's first pass (or
+    // the last finalize), then the wait until cycle i's counters are on the
+    // host: the next cycle's first event, or the end of the stream
+    auto finishCycle = [&](int i) {
+        if (i + 1 < nCycles) {
+            enqueueCycle(i + 1, false);
+            hipCheck(hipEventSynchronize(p->runEv[(i + 1) & 3].t0), "hipEventSynchronize");
+        } else {
+            enqueueFinalize(p);
+            hipCheck(hipEventRecord(p->runEnd, p->stream), "hipEventRecord");
+            hipCheck(hipEventSynchronize(p->runEnd), "hipEventSynchronize");
+        }
+    };
+    try {
+        hipCheck(hipEventRecord(p->runStart, p->stream), "hipEventRecord");
+        enqueueCycle(0, true);
+        hipEvent_t prevEnd = p->runStart;
+        for (int i = 0; i < nCycles; i++) {
+            auto& e = p->runEv[i & 3];
+            finishCycle(i);
+            const qs::CycleBlock* h = p->hcb[i & 1];
+            const bool held = h->ctl.hold != 0;
+            hipEvent_t end = e.p1;
+            if (held) {
+                // everything enqueued after this cycle's tail has returned at once
+                hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+                checkErrors(h->pc.error);
+                // the first pass left secondaries: the further passes, one by one
+                const uint32_t nExtra = h->pc.extraCount[0];
+                qs::launchClear(c, &p->ctl->hold, sizeof(uint32_t));
+                runPasses(p, 1, p->cur, p->extra[0].v, &p->pc->extraCount[0], nExtra, e.t1);
+                h = p->hcb[0];  // the counters after the last pass
+                end = e.t1;
+            }
+            p->stats.init_seconds += elapsed(prevEnd, e.t0);
+            p->stats.track_kernel_seconds += elapsed(e.t0, e.p1);
+            p->stats.track_launches += 1;
+            const double ts = elapsed(e.t0, end);
+            p->lastTrackingSeconds = ts;
+            p->stats.tracking_seconds += ts;
+            if (trackSec) trackSec[i] = ts;
+            p->nPop = h->pc.nPopOut;
+            closeCycle(p, h->dc, h->pc.nCensus, bals ? &bals[i] : nullptr, fluxSums ? &fluxSums[i] : nullptr);
+            if (held) {
+                // the held finalize and next cycle, again
+                if (i + 1 < nCycles) enqueueCycle(i + 1, true);
+                else finishCycle(i);
+            }
+            prevEnd = end;
+        }
+        p->stats.finalize_seconds += elapsed(prevEnd, p->runEnd);
+    } catch (...) {
+        // nothing still in flight may write into the caller's arrays later
+        (void)hipStreamSynchronize(p->stream);
+        throw;
+    }
 }
 
 qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
@@ -545,9 +635,9 @@ uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T) {
     const uint64_t nc = (uint64_t)T.nCells, G = (uint64_t)T.nGroups, M = T.materials.size();
     const uint64_t R = (uint64_t)T.maxReactions;
     uint64_t b = nc * 96 * 8 + nc * 4 + nc * 8 + nc * 24 + (nc + 1) * 8 + nc * G * 8;
-    b += (G + 1) * 8 + M * G * 8 + M * R * G * 8 + M * R * 4;
+    b += (G + 1) * 8 + 2 * M * G * 8 + M * R * G * 8 + M * R * 4;
     const uint64_t cap = particleCapacity(T);
-    b += 5 * (cap * qs::kParticleBytes + 12 * 256);
+    b += 5 * (cap * qs::kParticleBytes + 12 * 256) + qs::popGroups((uint32_t)cap) * sizeof(qs::PopGroup);
     b += nc * 24 * 16 + qs::sortKeys((int)nc) * 4 + qs::scanBlocks(qs::sortKeys((int)nc)) * 4;
     return b + 64 * 1024;
 }
@@ -670,24 +760,7 @@ int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum) {
 int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums, double* tracking_seconds) {
     QSG_GUARD({
         if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
-        // one host synchronisation per cycle (after each tracking pass): the
-        // set-up is not read back before tracking, and each flux sum is
-        // collected with the next cycle's synchronisation
-        try {
-            for (int i = 0; i < n_cycles; i++) {
-                cycleInit(p, true);
-                cycleTracking(p);
-                if (tracking_seconds) tracking_seconds[i] = p->lastTrackingSeconds;
-                cycleFinalize(p, balances ? &balances[i] : nullptr, flux_sums ? &flux_sums[i] : nullptr, true);
-            }
-            hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-            completeFinalize(p);
-        } catch (...) {
-            // nothing still in flight may write into the caller's arrays later
-            p->finalizePending = false;
-            p->popPending = false;
-            throw;
-        }
+        runCycles(p, n_cycles, balances, flux_sums, tracking_seconds);
     })
 }
 

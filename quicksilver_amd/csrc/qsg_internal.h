@@ -88,8 +88,6 @@ struct qsg_problem_s {
     int cur = 0;               // proc[cur] holds this cycle's processing population
     uint64_t startCount = 0;   // census particles carried into this cycle
     uint32_t nPop = 0;         // processing population after population control
-    bool popPending = false;   // cycle set-up enqueued, its counters not read yet
-    bool finalizePending = false;  // a finalize whose timing is still in flight
     uint32_t nCensus = 0;      // census count after tracking
     int censusBuf = -1;        // proc index holding the last census
     int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
@@ -97,17 +95,29 @@ struct qsg_problem_s {
 
     uint32_t* sortCounts = nullptr;     // [nCells * kSortBands]
     uint32_t* sortBlockSums = nullptr;  // scan scratch
+    qs::PopGroup* popGroups = nullptr;  // population-control scratch
     bool queueSorted = false;           // this cycle's queue is proc[cur] (sorted), not pop
 
+    qs::CycleBlock* cb = nullptr;     // device counters: pc, dc and ctl point into it
     qs::PassCounters* pc = nullptr;
     qs::DevCounters* dc = nullptr;
-    qs::PassCounters* hpc = nullptr;  // pinned host mirrors
-    qs::DevCounters* hdc = nullptr;
+    qs::CycleCtl* ctl = nullptr;
+    qs::CycleBlock* hcb[2] = {};      // pinned host copies (two cycles in flight in qsg_run)
+    qs::PassCounters* hpc = nullptr;  // = &hcb[0]->pc, the copy syncCounters fills
+    qs::DevCounters* hdc = nullptr;   // = &hcb[0]->dc
 
     qs::TrackTuning tune = qs::kDefaultTuning;
     qsgi::StackPool stacks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evI0 = nullptr, evI1 = nullptr, evF0 = nullptr, evF1 = nullptr;
     std::vector<hipEvent_t> passEv;  // begin/end pairs of tracking-pass launches
+    // events of the cycles qsg_run has in flight (a ring): before and after
+    // the first pass, and after the last pass of a cycle that needed more;
+    // each recorded event costs the stream about 10 us, so there are no more
+    struct RunEvents {
+        hipEvent_t t0, p1, t1;
+    };
+    RunEvents runEv[4] = {};
+    hipEvent_t runStart = nullptr, runEnd = nullptr;
     double lastTrackingSeconds = 0.0;
     qsg_stats stats = {};
     qsg_balance cumulative = {};  // Tallies::_balanceCumulative (Tallies.cc:65)

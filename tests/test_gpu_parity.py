@@ -125,27 +125,37 @@ def test_device_fp64_builtins_round_like_ieee():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["nested_small", "coral2_p2", "all_escape"])
-def test_run_matches_the_three_phase_calls(name):
-    """qsg_run synchronises once per cycle (the set-up's counters come back
-    with the first tracking pass, each flux sum with the next cycle); the
-    three per-cycle entry points synchronise after each phase.  Both must
-    give the same tallies bit for bit, the same stats counts, and flux sums
-    within the flux tolerance (the flux bins are FP64 atomic sums, whose
-    last bits depend on the order the device happens to add them in)."""
-    print(f"[parity] qsg_run vs cycle_init/tracking/finalize, deck={name}", flush=True)
+@pytest.mark.parametrize("name,shape", [("nested_small", "default"), ("coral2_p2", "default"),
+                                        ("all_escape", "default"), ("nested_small", "passes_only"),
+                                        ("spheres", "passes_only")])
+def test_run_matches_the_three_phase_calls(name, shape):
+    """qsg_run enqueues each cycle before it waits for the previous one (the
+    set-up reads the census count on the device); a cycle whose first pass
+    leaves secondaries holds the kernels enqueued after it, and the host
+    finishes it with further passes (every cycle with fission does so when
+    the wave stacks are off).  The three per-cycle entry points synchronise
+    after each phase.  Both must give the same tallies bit for bit, the same
+    stats counts, and flux sums within the flux tolerance (the flux bins are
+    FP64 atomic sums, whose last bits depend on the order the device
+    happens to add them in)."""
+    print(f"[parity] qsg_run vs cycle_init/tracking/finalize, deck={name} shape={shape}", flush=True)
     g = load_golden(name)
     n = len(g["cycles"])
+    knobs = SHAPES.get(shape, {})
     with q.Problem.from_file(g["deck_path"]) as p:
+        p.set_tuning(**knobs)
         fast = p.run(n)
         st_fast = p.stats()
     steps = []
     with q.Problem.from_file(g["deck_path"]) as p:
+        p.set_tuning(**knobs)
         for _ in range(n):
             p.cycle_init()
             p.cycle_tracking()
             steps.append(p.cycle_finalize())
         st_step = p.stats()
+    if shape == "passes_only":
+        assert st_fast["track_launches"] > st_fast["cycles"], "no cycle needed a second pass"
     for c, ((b1, f1, _), (b2, f2)) in enumerate(zip(fast, steps)):
         assert b1 == b2, f"cycle {c}: {b1} vs {b2}"
         assert abs(f1 - f2) <= FLUX_RTOL * abs(f2), f"cycle {c} flux {f1!r} vs {f2!r}"

@@ -109,10 +109,15 @@ int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum);
 
 /* Runs n cycles; per-cycle results into arrays of length n (may be NULL).
  * tracking_seconds (may be NULL) receives the device time of the tracking
- * phase of each cycle.  The same results as n calls of the three phases,
- * with one host synchronisation per cycle instead of four; the arrays are
- * complete when the call returns (on an error, entries of the failing cycle
- * may be unset). */
+ * phase of each cycle.  The same results as n calls of the three phases.
+ * Each cycle is enqueued before the host waits for the previous one: the
+ * set-up reads the carried census count on the device, so the device goes
+ * from cycle to cycle without waiting for the host.  A cycle whose first
+ * tracking pass leaves secondaries (a wave stack overflowed, or
+ * stack_entries is 0) or reports an error makes every kernel enqueued after
+ * it return at once; the host then finishes that cycle with further passes
+ * and enqueues the next one again.  The arrays are complete when the call
+ * returns (on an error, entries of the failing cycle may be unset). */
 int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sums,
             double* tracking_seconds);
 
@@ -124,8 +129,12 @@ typedef struct qsg_stats {
                                     half of each step, which ends with the
                                     first pass)                              */
     double tracking_seconds;     /* cycle tracking phases, first to last pass */
-    double init_seconds;         /* source + population control              */
-    double finalize_seconds;     /* flux reduction + counter readback        */
+    double init_seconds;         /* source + population control; under
+                                    qsg_run all device time between two
+                                    cycles' tracking (counter copy, finalize
+                                    and set-up)                              */
+    double finalize_seconds;     /* fluence + flux reset (+ counter readback);
+                                    under qsg_run the last cycle's only      */
     uint64_t track_launches;     /* tracking-pass kernel launches            */
     uint64_t segments;           /* segments tracked                         */
     uint64_t collisions, facet_crossings, particle_loads, cycles;

@@ -1786,22 +1786,33 @@ __device__ __forceinline__ void pop_write(const DevProblem& P, const Particles& 
 }
 
 // One group of a batch problem's population control: the group's output
-// base is one atomic on nPopOut and the roulette and split counts one atomic
-// per wave.  Called by all threads of a block, thread t for input i
-// (valid: t holds an input; the inputs of a call are consecutive).
+// base is one atomic on nPopOut, and the roulette and split counts one
+// atomic each per group.  Called by all threads of a block (at most 4
+// waves), thread t for input i (valid: t holds an input; the inputs of a
+// call are consecutive).
 __device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
                                           int mode, double factor, const Particles& out, uint32_t outCap,
                                           uint32_t* nPopOut, uint32_t* error, DevCounters* dc) {
-    __shared__ uint32_t sBase;
+    __shared__ uint32_t sBase, sKill[4], sSplit[4];
     const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
     const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
     uint32_t total;
     const uint32_t excl = block_excl_scan(surv, total);
-    if (threadIdx.x == 0) sBase = total ? atomicAdd(nPopOut, total) : 0u;
     const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
     if (__lane_id() == 0) {
-        if (kills) atomicAdd(&dc->rr, (unsigned long long)kills);
-        if (splits) atomicAdd(&dc->split, (unsigned long long)splits);
+        sKill[threadIdx.x >> 6] = kills;
+        sSplit[threadIdx.x >> 6] = splits;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sBase = total ? atomicAdd(nPopOut, total) : 0u;
+        uint32_t k = 0, sp = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
+            k += sKill[w];
+            sp += sSplit[w];
+        }
+        if (k) atomicAdd(&dc->rr, (unsigned long long)k);
+        if (sp) atomicAdd(&dc->split, (unsigned long long)sp);
     }
     __syncthreads();
     pop_write(P, in, i, d, surv, excl, total, sBase, out, outCap, error);
@@ -2201,7 +2212,7 @@ __global__ void k_batch_prep(BatchSlot* slots, int n, uint64_t* srcPrefix, uint6
             srcPrefix[p] = a;
             popPrefix[p] = b;
             if (slots[p].st->active && slots[p].st->error == 0) a += slots[p].sourcePerCycle;
-            b += ((uint64_t)slots[p].st->nProc + 255) / 256 * 256;  // whole blocks per problem
+            b += ((uint64_t)slots[p].st->nProc + 255) / 256;  // population groups of 256 inputs
         }
         srcPrefix[n] = a;
         popPrefix[n] = b;
@@ -2230,16 +2241,18 @@ __global__ void __launch_bounds__(256) k_batch_source(BatchSlot* slots, int n, c
     }
 }
 
-// Population control of every problem of a batch: each problem's inputs
-// occupy whole 256-input blocks of the flat index space (popPrefix counts
-// them rounded up to 256), so one block serves one problem.
+// Population control of every problem of a batch: popPrefix counts each
+// problem's groups of 256 consecutive inputs, and blocks stride over the
+// groups of all problems, one group at a time.  (Smaller groups, as the
+// one-problem form uses when splitting, take more atomics on each
+// problem's output count and measured twice as slow here.)
 __global__ void __launch_bounds__(256) k_batch_population(BatchSlot* slots, int n, const uint64_t* popPrefix) {
     const uint64_t total = popPrefix[n];
-    for (uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x; b0 < total; b0 += (uint64_t)gridDim.x * blockDim.x) {
-        const int p = find_problem(popPrefix, n, b0);
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const int p = find_problem(popPrefix, n, g);
         const BatchSlot& S = slots[p];
         BatchState* st = S.st;
-        const uint32_t i = (uint32_t)(b0 - popPrefix[p]) + threadIdx.x;
+        const uint32_t i = (uint32_t)(g - popPrefix[p]) * 256u + threadIdx.x;
         pop_block(S.prob, S.proc[st->cycle & 1], i, i < st->nProc, (int)st->mode, st->factor, S.pop, S.cap,
                   &st->nPop, &st->error, S.dc);
     }

@@ -1711,11 +1711,12 @@ __device__ __forceinline__ uint64_t pop_next_survivor(const PopDecision& d, doub
 
 // Population control with the outputs written in order: one thread decides
 // for each input particle (pop_decide), a block scan gives each input its
-// output range, and the block writes its outputs a tile and a field at a
-// time through LDS, so every store instruction covers consecutive slots
+// output range, and the block stages its outputs a tile at a time in LDS
+// and stores them field by field, so every store instruction covers
+// consecutive slots
 // (one thread writing its ~factor copies one after another touches factor
 // times more cache lines per store).
-constexpr uint32_t kPopTile = 1024;
+constexpr uint32_t kPopTile = 256;  // outputs staged per round (12 fields: 24 KiB of LDS)
 constexpr uint32_t kPopMinGroup = 64;  // fewest inputs per block-wide population group
 
 // Exclusive scan of v over the block (at most 16 waves); total receives the
@@ -1737,12 +1738,14 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 }
 
 // Writes the outputs of one group of inputs: thread t's surv outputs go to
-// out[base + excl ...]; total is the group's output count.  Slots at or
-// beyond outCap are not written (error bit 4).
+// out[base + excl ...]; total is the group's output count.  The outputs are
+// staged a tile at a time, all twelve fields at once, in LDS, then stored
+// field by field over consecutive slots.  Slots at or beyond outCap are not
+// written (error bit 4).
 __device__ __forceinline__ void pop_write(const DevProblem& P, const Particles& in, uint32_t i, const PopDecision& d,
                                           uint32_t surv, uint32_t excl, uint32_t total, uint32_t base,
                                           const Particles& out, uint32_t outCap, uint32_t* error) {
-    __shared__ double sBuf[kPopTile];
+    __shared__ double sBuf[12][kPopTile];
     const double cutoff = P.lowWeightCutoff;
     double f[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int32_t cell = 0;
@@ -1757,31 +1760,27 @@ __device__ __forceinline__ void pop_write(const DevProblem& P, const Particles& 
     for (uint32_t t0 = 0; t0 < total; t0 += kPopTile) {
         const uint32_t t1 = min(t0 + kPopTile, total);
         const uint32_t lo = max(excl, t0), hi = min(excl + surv, t1);
-        // fields 0-9 (the weight is field 7), then the seed, then the cell
-        for (int k = 0; k < 12; k++) {
-            if (k < 10) {
-                for (uint32_t o = lo; o < hi; o++) sBuf[o - t0] = f[k];
-            } else if (k == 10) {
-                for (uint32_t o = lo; o < hi; o++)
-                    sBuf[o - t0] = __longlong_as_double((long long)pop_next_survivor(d, cutoff, st, j));
-            } else {
-                for (uint32_t o = lo; o < hi; o++) reinterpret_cast<int32_t*>(sBuf)[o - t0] = cell;
-            }
-            __syncthreads();
-            for (uint32_t q = threadIdx.x; q < t1 - t0; q += blockDim.x) {
-                const uint32_t o = base + t0 + q;
-                if (o >= outCap) {
-                    if (k == 0) atomicOr(error, 4u);
-                } else if (k < 10) {
-                    out.f(k)[o] = sBuf[q];
-                } else if (k == 10) {
-                    out.seed()[o] = (uint64_t)__double_as_longlong(sBuf[q]);
-                } else {
-                    out.cell()[o] = reinterpret_cast<const int32_t*>(sBuf)[q];
-                }
-            }
-            __syncthreads();
+        // fields 0-9 (the weight is field 7), the seed, the cell
+        for (uint32_t o = lo; o < hi; o++) {
+            const uint32_t q = o - t0;
+#pragma unroll
+            for (int k = 0; k < 10; k++) sBuf[k][q] = f[k];
+            sBuf[10][q] = __longlong_as_double((long long)pop_next_survivor(d, cutoff, st, j));
+            reinterpret_cast<int32_t*>(sBuf[11])[q] = cell;
         }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < t1 - t0; q += blockDim.x) {
+            const uint32_t o = base + t0 + q;
+            if (o >= outCap) {
+                atomicOr(error, 4u);
+                continue;
+            }
+#pragma unroll
+            for (int k = 0; k < 10; k++) out.f(k)[o] = sBuf[k][q];
+            out.seed()[o] = (uint64_t)__double_as_longlong(sBuf[10][q]);
+            out.cell()[o] = reinterpret_cast<const int32_t*>(sBuf[11])[q];
+        }
+        __syncthreads();
     }
 }
 
@@ -1880,39 +1879,47 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 // output count into nPopOut.  More outputs than outCap: error bit 4, hold,
 // and nPopOut stays at outCap.
 __global__ void __launch_bounds__(1024) k_pop_scan(CycleBlock* cb, PopGroup* __restrict__ groups, uint32_t outCap) {
-    __shared__ unsigned long long sK[16], sS[16];
+    __shared__ unsigned long long sK[16], sS[16], sT[16];
     CycleCtl& c = cb->ctl;
     if (c.hold) return;
     const uint32_t size = pop_group_size(c);
     const uint32_t nGroups = (c.n + size - 1) / size;
-    uint64_t carry = 0;
+    // thread t scans groups [t * per, (t + 1) * per) by itself, so one block
+    // scan of the threads' sums covers every group
+    const uint32_t per = (nGroups + blockDim.x - 1) / blockDim.x;
+    const uint32_t g0 = min(threadIdx.x * per, nGroups), g1 = min(g0 + per, nGroups);
+    uint64_t mine = 0;
     unsigned long long kills = 0, splits = 0;
-    for (uint32_t g0 = 0; g0 < nGroups; g0 += blockDim.x) {
-        const uint32_t g = g0 + threadIdx.x;
-        uint32_t v = 0;
-        if (g < nGroups) {
-            const PopGroup r = groups[g];
-            v = r.total;
-            kills += r.kills;
-            splits += r.splits;
-        }
-        uint32_t tot;
-        const uint32_t excl = block_excl_scan(v, tot);
-        if (g < nGroups) groups[g].base = (uint32_t)(carry + excl);
-        carry += tot;
+    for (uint32_t g = g0; g < g1; g++) {
+        const PopGroup r = groups[g];
+        mine += r.total;
+        kills += r.kills;
+        splits += r.splits;
     }
+    // the bases are exact while the outputs fit 32 bits; more than that
+    // exceeds the capacity, which the exact 64-bit total below reports
+    uint32_t tot;
+    const uint32_t excl = block_excl_scan(mine > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)mine, tot);
+    uint64_t run = excl;
+    for (uint32_t g = g0; g < g1; g++) {
+        groups[g].base = (uint32_t)run;
+        run += groups[g].total;
+    }
+    unsigned long long carry = wave_sum64(mine);
     kills = wave_sum64(kills);
     splits = wave_sum64(splits);
     const uint32_t wid = threadIdx.x >> 6;
     if (__lane_id() == 0) {
         sK[wid] = kills;
         sS[wid] = splits;
+        sT[wid] = carry;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
     for (uint32_t w = 1; w < (blockDim.x >> 6); w++) {
         kills += sK[w];
         splits += sS[w];
+        carry += sT[w];
     }
     cb->dc.rr += kills;
     cb->dc.split += splits;

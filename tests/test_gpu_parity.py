@@ -162,3 +162,44 @@ def test_run_matches_the_three_phase_calls(name, shape):
     for k in ("segments", "collisions", "facet_crossings", "particle_loads", "cycles", "track_launches"):
         assert st_fast[k] == st_step[k], k
     _check_against(g, [(b, f, 0.0) for b, f, _ in fast], "qsg_run")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,shape", [("nested_small", "default"), ("spheres", "sorted_every_cycle"),
+                                        ("coral2_p1", "default"), ("nonflat", "passes_only")])
+def test_runs_in_pieces_and_mixed_with_single_steps(name, shape):
+    """Cycles split over several qsg_run calls, with single-step cycles in
+    between, continue from one another: each run starts from the host's
+    census count and leaves its last cycle closed (its tail does no set-up),
+    and a single step afterwards starts from the same state."""
+    print(f"[parity] runs in pieces, deck={name} shape={shape}", flush=True)
+    g = load_golden(name)
+    n = len(g["cycles"])
+    out = []
+    with q.Problem.from_file(g["deck_path"]) as p:
+        p.set_tuning(**SHAPES.get(shape, {}))
+        pieces = [1, 2, 1]
+        done = 0
+        while done < n:
+            for k in pieces:
+                if done >= n:
+                    break
+                k = min(k, n - done)
+                out += p.run(k)
+                done += k
+                if done < n:
+                    p.cycle_init()
+                    p.cycle_tracking()
+                    bal, flux = p.cycle_finalize()
+                    out.append((bal, flux, 0.0))
+                    done += 1
+        assert p.stats()["cycles"] == n
+        fluence = p.fluence() if name.startswith("coral2") else None
+    _check_against(g, out, f"pieces/{shape}")
+    if fluence is not None:
+        # the cumulative fluence matches one uninterrupted run's
+        with q.Problem.from_file(g["deck_path"]) as p:
+            p.set_tuning(**SHAPES.get(shape, {}))
+            p.run(n)
+            whole = p.fluence()
+        np.testing.assert_allclose(fluence, whole, rtol=1e-10, atol=0.0)

@@ -144,7 +144,8 @@ struct TrackTuning {
 // Per-wave secondary stacks of a tracking launch.  Wave w (global wave
 // index) owns `cap` particle entries laid out like a particle buffer, at
 // base + w * 12 * cap * 8 with field stride cap * 8.  A wave pushes the
-// outputs of its multi-particle collisions there and pops them before it
+// secondaries of its multi-particle collisions there (the parent keeps its
+// lane) and pops them before it
 // takes new queue entries, so secondaries are tracked in the same launch
 // (depth first) instead of in a later pass.  Waves with index >= nWaves,
 // or cap == 0, send secondaries to the pass's extra queue instead.

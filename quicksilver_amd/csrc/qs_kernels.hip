@@ -1342,14 +1342,16 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 // Outputs: secondaries k = 1..nOut-1 are copies of the
                 // pre-collision parent with spawned seeds and outgoing pair k
                 // (CollisionEvent.cc:125-133); the parent takes pair 0 and the
-                // seed left after the spawns (CollisionEvent.cc:135).  A
-                // single survivor stays in the lane; several go to the wave's
-                // stack (or the pass's extra queue when the stack is full) as
-                // records whose updateTrajectory runs when they are loaded.
+                // seed left after the spawns (CollisionEvent.cc:135) and stays
+                // in the lane, as the reference's parent continues its
+                // history.  The secondaries go to the wave's stack (or the
+                // pass's extra queue when the stack is full) as records whose
+                // updateTrajectory runs when they are loaded.
                 const Particles dst = stackOk ? stack : io.extra;
                 const uint32_t dcap = stackOk ? scap : io.extraCap;
                 uint32_t slot = 0;
-                if (nOut > 1) slot = stackOk ? wave_push(sTopW, (uint32_t)nOut) : wave_append(io.nExtra, (uint32_t)nOut);
+                if (nOut > 1)
+                    slot = stackOk ? wave_push(sTopW, (uint32_t)(nOut - 1)) : wave_append(io.nExtra, (uint32_t)(nOut - 1));
 #pragma unroll 1
                 for (int k = 1; k <= nOut; k++) {
                     const bool parent = (k == nOut);
@@ -1371,8 +1373,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                         ek = (20 * rr * rr);
                         ak = rng_sample(t) * 2.0 - 1.0;
                     }
-                    if (nOut == 1) {
-                        // the survivor: updated at the top of the next iteration
+                    if (parent) {
+                        // the parent: updated at the top of the next iteration
                         L.E = ek;
                         L.nmfp = ak;
                         upd = true;
@@ -1383,7 +1385,6 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                         else atomicOr(io.error, 2u);
                     }
                 }
-                if (nOut > 1) active = false;
             }
         } else if (outcome == 1) {
             // ---- MC_Facet_Crossing_Event (MC_Facet_Crossing_Event.cc:25-70) ----

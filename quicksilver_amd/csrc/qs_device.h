@@ -123,6 +123,12 @@ struct CycleCtl {
     uint64_t pad;
 };
 
+// One group of consecutive population-control inputs: its output, roulette
+// and split counts, and the base of its outputs.
+struct PopGroup {
+    uint32_t total, kills, splits, base;
+};
+
 // A problem's counters in one block, so that one copy brings them back.
 struct CycleBlock {
     PassCounters pc;
@@ -190,6 +196,7 @@ struct BatchSlot {
     uint64_t nParticles;    // population target
     BatchState* st;
     DevCounters* dc;
+    PopGroup* groups;       // the problem's population-control scratch
 };
 
 // ---- launchers (qs_kernels.hip); all asynchronous on `stream` ----
@@ -222,9 +229,6 @@ void launchSource(const LaunchCtx& ctx, const DevProblem& P, Particles out, cons
 // in[0, cb->ctl.n) with ctl's mode and factor; inCap bounds ctl.n.  The
 // outputs go to out[0, cb->pc.nPopOut), the roulette and split counts to
 // cb->dc.  groups: scratch of popGroups(inCap) entries.
-struct PopGroup {
-    uint32_t total, kills, splits, base;
-};
 size_t popGroups(uint32_t inCap);
 void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, uint32_t inCap,
                       Particles out, uint32_t outCap, CycleBlock* cb, PopGroup* groups);

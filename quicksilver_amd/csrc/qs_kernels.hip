@@ -1785,39 +1785,6 @@ __device__ __forceinline__ void pop_write(const DevProblem& P, const Particles& 
     }
 }
 
-// One group of a batch problem's population control: the group's output
-// base is one atomic on nPopOut, and the roulette and split counts one
-// atomic each per group.  Called by all threads of a block (at most 4
-// waves), thread t for input i (valid: t holds an input; the inputs of a
-// call are consecutive).
-__device__ __forceinline__ void pop_block(const DevProblem& P, const Particles& in, uint32_t i, bool valid,
-                                          int mode, double factor, const Particles& out, uint32_t outCap,
-                                          uint32_t* nPopOut, uint32_t* error, DevCounters* dc) {
-    __shared__ uint32_t sBase, sKill[4], sSplit[4];
-    const PopDecision d = pop_decide(P, in, i, valid, mode, factor);
-    const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
-    uint32_t total;
-    const uint32_t excl = block_excl_scan(surv, total);
-    const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
-    if (__lane_id() == 0) {
-        sKill[threadIdx.x >> 6] = kills;
-        sSplit[threadIdx.x >> 6] = splits;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        sBase = total ? atomicAdd(nPopOut, total) : 0u;
-        uint32_t k = 0, sp = 0;
-        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
-            k += sKill[w];
-            sp += sSplit[w];
-        }
-        if (k) atomicAdd(&dc->rr, (unsigned long long)k);
-        if (sp) atomicAdd(&dc->split, (unsigned long long)sp);
-    }
-    __syncthreads();
-    pop_write(P, in, i, d, surv, excl, total, sBase, out, outCap, error);
-}
-
 // ---- one problem's population control: count, scan, write
 // Same-address atomics from every workgroup are slow on this device (they
 // serialise across the chiplets), so the one-problem form places each
@@ -2249,20 +2216,109 @@ __global__ void __launch_bounds__(256) k_batch_source(BatchSlot* slots, int n, c
     }
 }
 
-// Population control of every problem of a batch: popPrefix counts each
-// problem's groups of 256 consecutive inputs, and blocks stride over the
-// groups of all problems, one group at a time.  (Smaller groups, as the
-// one-problem form uses when splitting, take more atomics on each
-// problem's output count and measured twice as slow here.)
+// Population control of every problem of a batch, in the same three steps
+// as the one-problem form (no atomics on a problem's counters): popPrefix
+// counts each problem's groups of 256 consecutive inputs, and blocks stride
+// over the groups of all problems, one group at a time; k_batch_pop_scan
+// runs one workgroup per problem.
+__global__ void __launch_bounds__(256) k_batch_pop_count(BatchSlot* slots, int n, const uint64_t* popPrefix) {
+    __shared__ uint32_t sSum[3][4];
+    const uint64_t total = popPrefix[n];
+    const uint32_t wid = threadIdx.x >> 6;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const int p = find_problem(popPrefix, n, g);
+        const BatchSlot& S = slots[p];
+        const BatchState* st = S.st;
+        const uint32_t lg = (uint32_t)(g - popPrefix[p]);
+        const uint32_t i = lg * 256u + threadIdx.x;
+        const bool valid = i < st->nProc;
+        const PopDecision d = pop_decide(S.prob, S.proc[st->cycle & 1], i, valid, (int)st->mode, st->factor);
+        const uint32_t surv = wave_sum((d.keep && valid) ? d.survivors : 0u);
+        const uint32_t kills = wave_sum(d.kills), splits = wave_sum(d.splits);
+        if (__lane_id() == 0) {
+            sSum[0][wid] = surv;
+            sSum[1][wid] = kills;
+            sSum[2][wid] = splits;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            PopGroup r = {0u, 0u, 0u, 0u};
+            for (uint32_t w = 0; w < (blockDim.x >> 6); w++) {
+                r.total += sSum[0][w];
+                r.kills += sSum[1][w];
+                r.splits += sSum[2][w];
+            }
+            S.groups[lg] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// Block p: the scan of problem p's groups into output bases, its output
+// count into nPop (at most its capacity; more is error bit 4), and its
+// roulette and split counts into its balance.
+__global__ void __launch_bounds__(1024) k_batch_pop_scan(BatchSlot* slots, const uint64_t* popPrefix) {
+    __shared__ unsigned long long sK[16], sS[16], sT[16];
+    const int p = blockIdx.x;
+    const BatchSlot& S = slots[p];
+    BatchState* st = S.st;
+    const uint32_t nGroups = (uint32_t)(popPrefix[p + 1] - popPrefix[p]);
+    if (nGroups == 0) return;
+    PopGroup* groups = S.groups;
+    const uint32_t per = (nGroups + blockDim.x - 1) / blockDim.x;
+    const uint32_t g0 = min(threadIdx.x * per, nGroups), g1 = min(g0 + per, nGroups);
+    uint64_t mine = 0;
+    unsigned long long kills = 0, splits = 0;
+    for (uint32_t g = g0; g < g1; g++) {
+        const PopGroup r = groups[g];
+        mine += r.total;
+        kills += r.kills;
+        splits += r.splits;
+    }
+    uint32_t tot;
+    const uint32_t excl = block_excl_scan(mine > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)mine, tot);
+    uint64_t run = excl;
+    for (uint32_t g = g0; g < g1; g++) {
+        groups[g].base = (uint32_t)run;
+        run += groups[g].total;
+    }
+    unsigned long long carry = wave_sum64(mine);
+    kills = wave_sum64(kills);
+    splits = wave_sum64(splits);
+    const uint32_t wid = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        sK[wid] = kills;
+        sS[wid] = splits;
+        sT[wid] = carry;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (uint32_t w = 1; w < (blockDim.x >> 6); w++) {
+        kills += sK[w];
+        splits += sS[w];
+        carry += sT[w];
+    }
+    S.dc->rr += kills;
+    S.dc->split += splits;
+    st->nPop = carry < S.cap ? (uint32_t)carry : S.cap;
+    if (carry > S.cap) st->error |= 4u;
+}
+
 __global__ void __launch_bounds__(256) k_batch_population(BatchSlot* slots, int n, const uint64_t* popPrefix) {
     const uint64_t total = popPrefix[n];
     for (uint64_t g = blockIdx.x; g < total; g += gridDim.x) {
         const int p = find_problem(popPrefix, n, g);
         const BatchSlot& S = slots[p];
         BatchState* st = S.st;
-        const uint32_t i = (uint32_t)(g - popPrefix[p]) * 256u + threadIdx.x;
-        pop_block(S.prob, S.proc[st->cycle & 1], i, i < st->nProc, (int)st->mode, st->factor, S.pop, S.cap,
-                  &st->nPop, &st->error, S.dc);
+        const uint32_t lg = (uint32_t)(g - popPrefix[p]);
+        const uint32_t i = lg * 256u + threadIdx.x;
+        const bool valid = i < st->nProc;
+        const Particles in = S.proc[st->cycle & 1];
+        const PopDecision d = pop_decide(S.prob, in, i, valid, (int)st->mode, st->factor);
+        const uint32_t surv = (d.keep && valid) ? d.survivors : 0u;
+        uint32_t tot;
+        const uint32_t excl = block_excl_scan(surv, tot);
+        pop_write(S.prob, in, i, d, surv, excl, tot, S.groups[lg].base, S.pop, S.cap, &st->error);
     }
 }
 
@@ -2529,8 +2585,10 @@ void launchBatchSource(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint
 }
 
 void launchBatchPopulation(const LaunchCtx& ctx, BatchSlot* slots, int n, const uint64_t* popPrefix) {
-    hipLaunchKernelGGL(k_batch_population, dim3(ctx.numCUs * 8), dim3(256), 0, (hipStream_t)ctx.stream, slots, n,
-                       popPrefix);
+    hipStream_t s = (hipStream_t)ctx.stream;
+    hipLaunchKernelGGL(k_batch_pop_count, dim3(ctx.numCUs * 8), dim3(256), 0, s, slots, n, popPrefix);
+    hipLaunchKernelGGL(k_batch_pop_scan, dim3((unsigned)n), dim3(1024), 0, s, slots, popPrefix);
+    hipLaunchKernelGGL(k_batch_population, dim3(ctx.numCUs * 8), dim3(256), 0, s, slots, n, popPrefix);
 }
 
 void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind) {

@@ -144,6 +144,7 @@ void setMember(qsg_batch_s* b, int i, qsg_problem_s* p) {
         slot.loadBalance = (uint32_t)p->tables.loadBalance;
         slot.sourcePerCycle = p->tables.sourcePerCycle;
         slot.nParticles = p->tables.nParticles;
+        slot.groups = p->popGroups;
         st.nCensus = (uint32_t)p->startCount;
         st.cycle = (uint32_t)p->cycle;
         st.active = 1;

@@ -203,3 +203,21 @@ def test_runs_in_pieces_and_mixed_with_single_steps(name, shape):
             p.run(n)
             whole = p.fluence()
         np.testing.assert_allclose(fluence, whole, rtol=1e-10, atol=0.0)
+
+
+@pytest.mark.gpu
+def test_run_stats_are_consistent():
+    """qsg_run's device timings: every cycle's tracking time is positive and
+    covers its kernel time; the time between cycles counts as set-up and the
+    last cycle's copy and finalize as finalize."""
+    g = load_golden("nested_small")
+    n = len(g["cycles"])
+    with q.Problem.from_file(g["deck_path"]) as p:
+        out = p.run(n)
+        st = p.stats()
+    assert st["cycles"] == n and st["track_launches"] >= n
+    assert all(t > 0.0 for _, _, t in out)
+    assert st["init_seconds"] > 0.0 and st["finalize_seconds"] > 0.0
+    assert st["track_kernel_seconds"] <= st["tracking_seconds"] * (1 + 1e-6)
+    assert abs(sum(t for _, _, t in out) - st["tracking_seconds"]) <= 1e-9 + 1e-6 * st["tracking_seconds"]
+    assert st["segments"] == sum(c["num_segments"] for c in g["cycles"])

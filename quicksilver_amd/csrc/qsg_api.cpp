@@ -80,6 +80,13 @@ void copyParticles(const qs::Particles& d, const qsg_particles* h, uint64_t n, h
 
 void setDevice(qsg_problem_s* p) { hipCheck(hipSetDevice(p->device), "hipSetDevice"); }
 
+// Events that only time launches: recording one without the system-scope
+// fence (cache write-back and invalidation) leaves no gap of several
+// microseconds in the stream.  Nothing that the host reads depends on them:
+// counters come back through copies the host synchronises with, or through
+// the cycle tail, which fences its own stores to host memory.
+constexpr unsigned kTimingEvent = hipEventDisableSystemFence;
+
 // Enqueues the copy of every counter into the pinned host block `slot`.
 void copyCounters(qsg_problem_s* p, int slot) {
     hipCheck(hipMemcpyAsync(p->hcb[slot], p->cb, sizeof(qs::CycleBlock), hipMemcpyDeviceToHost, p->stream),
@@ -220,7 +227,8 @@ void build(qsg_problem_s* p, int device) {
     p->hpc = &p->hcb[0]->pc;
     p->hdc = &p->hcb[0]->dc;
     for (auto& r : p->runEv)
-        for (hipEvent_t* e : {&r.t0, &r.p1, &r.t1}) hipCheck(hipEventCreate(e), "hipEventCreate");
+        for (hipEvent_t* e : {&r.t0, &r.p1, &r.t1})
+            hipCheck(hipEventCreateWithFlags(e, kTimingEvent), "hipEventCreateWithFlags");
     for (hipEvent_t* e : {&p->runStart, &p->runEnd}) hipCheck(hipEventCreate(e), "hipEventCreate");
     // the uploads and clears above ran on the null stream, which the
     // problem's non-blocking stream does not wait for; other devices' and
@@ -311,7 +319,7 @@ int runPasses(qsg_problem_s* p, int k, int cur, qs::Particles in, const uint32_t
     while (nInHost > 0) {
         while ((int)p->passEv.size() < 2 * (q + 1)) {
             hipEvent_t e;
-            hipCheck(hipEventCreate(&e), "hipEventCreate");
+            hipCheck(hipEventCreateWithFlags(&e, kTimingEvent), "hipEventCreateWithFlags");
             p->passEv.push_back(e);
         }
         hipCheck(hipEventRecord(p->passEv[2 * q], p->stream), "hipEventRecord");

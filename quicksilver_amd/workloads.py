@@ -51,53 +51,82 @@ def build_problems(texts: Sequence[str], tuning: Optional[Dict[str, int]] = None
     return probs
 
 
-def run_one_at_a_time(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None) -> Dict:
+def run_one_at_a_time(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None,
+                      cycles: Optional[int] = None, warmup: int = 0) -> Dict:
     """Each deck alone: every step of one problem, then the next.  The
     problems are constructed up front, the same way as for run_batched, and
-    each is released after its run.  solve_s counts only the steps (what the
-    batched form is compared on); wall_s also counts construction and
-    release."""
+    each is released after its run.  Each problem runs `warmup` untimed
+    cycles, then `cycles` timed ones (default: the deck's nSteps).  solve_s
+    counts only the timed cycles (what the batched form is compared on);
+    wall_s also counts construction, warm-up and release.  balances /
+    fluxes hold the timed cycles; kernel_seconds is the tracking kernels'
+    device time over them."""
     t0 = time.perf_counter()
     segments = 0
     solve = 0.0
+    warm = 0.0
+    kernel = 0.0
     per = []
+    fluxes = []
     probs = build_problems([d.text() for d in decks], tuning)
     try:
-        for p in probs:
+        for p, d in zip(probs, decks):
+            if warmup:
+                w0 = time.perf_counter()
+                p.run(warmup)
+                warm += time.perf_counter() - w0
+            p.reset_stats()
+            n = int(d.simulation["nSteps"]) if cycles is None else cycles
             s0 = time.perf_counter()
-            out = p.run()
+            out = p.run(n)
             solve += time.perf_counter() - s0
+            kernel += p.stats()["track_kernel_seconds"]
             p.close()
             per.append([b for b, _, _ in out])
+            fluxes.append([f for _, f, _ in out])
             segments += sum(b["num_segments"] for b, _, _ in out)
     finally:
         for p in probs:
             p.close()
     wall = time.perf_counter() - t0
     return {"segments": segments, "wall_s": wall, "solve_s": solve, "segments_per_s": segments / solve,
-            "end_to_end_segments_per_s": segments / wall, "balances": per}
+            "end_to_end_segments_per_s": segments / (wall - warm), "kernel_seconds": kernel,
+            "balances": per, "fluxes": fluxes}
 
 
-def run_batched(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None) -> Dict:
-    """All decks in one batch; solve_s counts the batch steps only."""
+def run_batched(decks: Sequence[Deck], tuning: Optional[Dict[str, int]] = None,
+                cycles: Optional[int] = None, warmup: int = 0) -> Dict:
+    """All decks in one batch: `warmup` untimed steps, then the timed ones
+    (`cycles` steps of every problem, default each deck's own nSteps, each
+    problem leaving the batch when it is done).  solve_s counts the timed
+    steps only; stats are the batch's device timings over them."""
     t0 = time.perf_counter()
     probs = build_problems([d.text() for d in decks])
     try:
         with q.Batch(probs) as b:
             if tuning:
                 b.set_tuning(**tuning)
+            t1 = time.perf_counter()
+            for _ in range(warmup):
+                b.step()
+            st0 = b.stats()
+            steps = [int(d.simulation["nSteps"]) if cycles is None else cycles for d in decks]
             s0 = time.perf_counter()
-            out = b.run([int(d.simulation["nSteps"]) for d in decks])
+            out = b.run(steps)
             solve = time.perf_counter() - s0
-            st = b.stats()
+            st1 = b.stats()
+            st = {k: st1[k] - st0[k] for k in st1}
     finally:
+        r0 = time.perf_counter()
         for p in probs:
             p.close()
     wall = time.perf_counter() - t0
     segments = sum(bal["num_segments"] for per in out for bal, _ in per)
     return {"segments": segments, "wall_s": wall, "solve_s": solve, "segments_per_s": segments / solve,
-            "end_to_end_segments_per_s": segments / wall,
-            "balances": [[bal for bal, _ in per] for per in out], "stats": st}
+            "end_to_end_segments_per_s": segments / (wall - (s0 - t1)), "steps": max(steps) if steps else 0,
+            "build_s": t1 - t0, "warmup_s": s0 - t1, "release_s": time.perf_counter() - r0,
+            "balances": [[bal for bal, _ in per] for per in out],
+            "fluxes": [[f for _, f in per] for per in out], "stats": st}
 
 
 def _percentiles(lat: List[float]) -> Dict[str, float]:

@@ -21,8 +21,18 @@ def load_golden(name):
     return g
 
 
+# fixture files holding many generated decks' tallies (not one deck each)
+WORKLOAD_SETS = ("mixed_batch", "serve_stream")
+
+
 def golden_names():
-    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
+    """Single-deck fixtures (tests/golden/<name>.json with decks/<name>.inp)."""
+    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json") and f[:-5] not in WORKLOAD_SETS)
+
+
+def load_workload_set(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
 
 
 @pytest.fixture(scope="session")

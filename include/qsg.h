@@ -12,6 +12,8 @@
  *   qsg_create / qsg_create_from_deck   getParameters  Parameters.cc:80
  *                                       + initMC       initMC.cc:53
  *   qsg_cycle_init                      cycleInit      main.cc:96-121
+ *   qsg_cycle_begin_host                the end of cycleInit (main.cc:117-120): the
+ *                                       processing vaults the host filled
  *   qsg_cycle_tracking                  cycleTracking  main.cc:138-307
  *                                       (CycleTrackingGuts, CycleTracking.cc:15)
  *   qsg_cycle_finalize                  cycleFinalize  main.cc:310-324
@@ -95,14 +97,45 @@ int qsg_create(int argc, const char* const* argv, int device, qsg_problem* out);
 /* Deck given as text (optional extra argv applied first, like the CLI). */
 int qsg_create_from_deck(const char* deck_text, int argc, const char* const* argv, int device,
                          qsg_problem* out);
+
+/* Per-problem sizing.  No reference counterpart: the reference's vaults grow
+ * on the host.  particle_capacity is the number of entries of each of the
+ * problem's particle buffers (processing, census, population-control output
+ * and the two secondary buffers); 0 selects the default, 2 x nParticles + one
+ * cycle's source + 65536.  A cycle that would need more stops with
+ * QSG_ERR_CAPACITY and leaves the problem failed (see below). */
+typedef struct qsg_options {
+    uint64_t particle_capacity;
+} qsg_options;
+/* qsg_create_from_deck with options (opts may be NULL: the defaults). */
+int qsg_create_from_deck_opts(const char* deck_text, int argc, const char* const* argv, int device,
+                              const qsg_options* opts, qsg_problem* out);
 int qsg_destroy(qsg_problem p);
 
 int qsg_get_info(qsg_problem p, qsg_info* info);
 int qsg_set_tuning(qsg_problem p, const qsg_tuning* t);
 int qsg_get_tuning(qsg_problem p, qsg_tuning* t);
 
+/* A problem whose cycle stopped part way (a device error flag: a buffer
+ * overflow, a runaway history; or a HIP failure) is failed: its device state
+ * is no longer a cycle boundary, so every later call that would advance or
+ * read its particles returns QSG_ERR_STATE.  qsg_problem_error gives the
+ * code and message it failed with (code 0: not failed).  Errors found before
+ * any device work (arguments, call order, a population that cannot fit)
+ * leave the problem usable. */
+int qsg_problem_error(qsg_problem p, int* code, char* message, size_t message_len);
+
 /* Per-cycle phases, in order. */
 int qsg_cycle_init(qsg_problem p);
+/* In place of qsg_cycle_init, for a caller that keeps the reference's own
+ * cycleInit (MC_SourceNow, PopulationControl, RouletteLowWeightParticles on
+ * the host, main.cc:96-121): the n given particles become this cycle's
+ * processing population and no device source or population control runs.
+ * Then qsg_cycle_tracking, qsg_get_census / qsg_get_scalar_flux and
+ * qsg_cycle_finalize as usual; the cycle's balance reports start = n,
+ * source = rr = split = 0 (the caller booked those) and the tracking
+ * counters.  Particle cells are global ids (i + nx*(j + ny*k)). */
+int qsg_cycle_begin_host(qsg_problem p, const qsg_particles* in, uint64_t n);
 int qsg_cycle_tracking(qsg_problem p);
 /* Writes this cycle's balance and scalar-flux sum, then zeroes the flux. */
 int qsg_cycle_finalize(qsg_problem p, qsg_balance* balance, double* flux_sum);
@@ -124,11 +157,9 @@ int qsg_run(qsg_problem p, int n_cycles, qsg_balance* balances, double* flux_sum
 /* Device timings accumulated since creation (or the last qsg_reset_stats),
  * measured with HIP events on the problem's stream. */
 typedef struct qsg_stats {
-    double track_kernel_seconds; /* sum over tracking-pass kernel launches (a
-                                    batch replaying graphs: the captured first
-                                    half of each step, which ends with the
-                                    first pass)                              */
-    double tracking_seconds;     /* cycle tracking phases, first to last pass */
+    double track_kernel_seconds; /* sum over tracking-pass kernel launches    */
+    double tracking_seconds;     /* cycle tracking phases, first to last pass
+                                    (a batch: each whole step)               */
     double init_seconds;         /* source + population control; under
                                     qsg_run all device time between two
                                     cycles' tracking (counter copy, finalize
@@ -166,8 +197,17 @@ int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n);
 typedef struct qsg_batch_s* qsg_batch;
 int qsg_batch_create(qsg_problem* problems, int n, qsg_batch* out);
 int qsg_batch_destroy(qsg_batch b);
-/* balances / flux_sums: arrays of n (may be NULL); inactive problems get 0 */
+/* balances / flux_sums: arrays of n (may be NULL); inactive problems get 0.
+ * A problem whose cycle raises a device error flag (QSG_ERR_CAPACITY,
+ * QSG_ERR_NUMERIC) is taken out of the batch and failed; every other problem
+ * finishes the step as usual and its entries are valid.  The step then
+ * returns the first such problem's code, and qsg_batch_problem_error names
+ * each failed problem.  Any other failure (a HIP error) fails the whole
+ * batch: its later steps return QSG_ERR_STATE. */
 int qsg_batch_step(qsg_batch b, qsg_balance* balances, double* flux_sums);
+/* code: 0, or the error a problem was taken out of the batch with. */
+int qsg_batch_problem_error(qsg_batch b, int index, int* code);
+/* A failed problem cannot be made active again (QSG_ERR_STATE). */
 int qsg_batch_set_active(qsg_batch b, int index, int active);
 int qsg_batch_set_tuning(qsg_batch b, const qsg_tuning* t);
 int qsg_batch_get_stats(qsg_batch b, qsg_stats* stats);
@@ -178,8 +218,10 @@ int qsg_batch_get_stats(qsg_batch b, qsg_stats* stats);
  * cap), kept resident on the device, advanced together one cycle per
  * qsg_server_step (requests take and free slots of one running batch), and retired
  * after their nSteps cycles.  A request's per-cycle tallies equal those of the
- * same deck run alone.  No reference counterpart.
- * Request states: 0 queued, 1 running, 2 done, 3 failed. */
+ * same deck run alone.  A request whose cycle fails (a buffer overflow, a
+ * runaway history) is failed alone; the others continue.  No reference
+ * counterpart.
+ * Request states: 0 queued, 1 running, 2 done, 3 failed, 4 cancelled. */
 typedef struct qsg_server_s* qsg_server;
 typedef struct qsg_server_stats {
     uint64_t submitted, admitted, completed, rejected, steps, segments;
@@ -188,9 +230,19 @@ typedef struct qsg_server_stats {
 int qsg_server_create(int device, uint64_t hbm_budget_bytes, int max_active, qsg_server* out);
 int qsg_server_destroy(qsg_server s);
 int qsg_server_submit(qsg_server s, const char* deck_text, uint64_t* request_id);
+/* qsg_server_submit with per-request sizing (opts may be NULL). */
+int qsg_server_submit_opts(qsg_server s, const char* deck_text, const qsg_options* opts, uint64_t* request_id);
 int qsg_server_step(qsg_server s, int* n_active, int* n_queued);
 int qsg_server_status(qsg_server s, uint64_t id, int* state, int* cycles_done);
+/* The first n completed cycles of a request (also of a running or cancelled
+ * one).  A failed request returns the code it failed with (the message in
+ * qsg_last_error). */
 int qsg_server_result(qsg_server s, uint64_t id, qsg_balance* balances, double* flux_sums, int n);
+/* Stops a queued or running request between steps (every running request
+ * is then at a cycle boundary): its slot and device memory are freed at once
+ * and its state becomes 4; the cycles it completed stay readable. */
+int qsg_server_cancel(qsg_server s, uint64_t id);
+/* Forgets a request (a running one is cancelled first). */
 int qsg_server_release(qsg_server s, uint64_t id);
 int qsg_server_set_tuning(qsg_server s, const qsg_tuning* t);
 int qsg_server_get_stats(qsg_server s, qsg_server_stats* stats);
@@ -213,6 +265,16 @@ typedef struct qsg_coral_report {
     double fluence_max_percent_diff;
 } qsg_coral_report;
 int qsg_coral_check(qsg_problem p, qsg_coral_report* report);
+/* Census energy spectrum (EnergySpectrum, EnergySpectrum.cc:12-59): over all
+ * cycles so far, the census particles of each cycle counted per energy group
+ * (counts[G]: energies above eMax); energies receives the group bounds
+ * (n_groups + 1 values, may be NULL).  n = n_groups + 1.  Kept only when the
+ * command line or deck sets energySpectrum (-e); the qs driver writes it to
+ * <energySpectrum>.dat at the end of the run, as gameOver does (main.cc:93).
+ * (crossSectionsOut (-S) is written by qsg_create / qsg_create_from_deck,
+ * as initMC does, initMC.cc:72.) */
+int qsg_get_energy_spectrum(qsg_problem p, double* energies, uint64_t* counts, uint64_t n);
+
 /* Cumulative per-cell fluence (n_cells values, global cell order); only for
  * decks that set coralBenchmark. */
 int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells);

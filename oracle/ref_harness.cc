@@ -28,7 +28,20 @@
 //                          (coralBenchmark decks), f64, global cell order.
 //   QSREF_CENSUS_DUMP=path write the census particles of the last cycle as
 //                          text: seed x y z vx vy vz E w tc nmfp cell
+//   QSREF_PRINT_PARAMS=1   print the parameters first, as main.cc:44 does,
+//                          then a line QSREF_PARAMS_END
+// At the end the census energy spectrum is written when energySpectrum (-e)
+// is set, as gameOver does (main.cc:93); crossSectionsOut (-S) is written by
+// initMC itself (initMC.cc:72).
+//
+// Built with -DQSREF_GPU_TRACKING (oracle/_ref/qs_ref_gpu), the same driver
+// calls the drop-in cycleTracking(MonteCarlo*) of
+// quicksilver_amd/shim/qsg_cycle_tracking.cc instead of the reference's
+// tracking loop: everything else (source, population control, tallies,
+// printout, CORAL2 checks) is still the reference's own code, so its output
+// against the plain build's checks the drop-in on the reference's terms.
 #include <chrono>
+#include <iostream>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -55,6 +68,10 @@
 
 #ifdef HAVE_OPENMP
 #include <omp.h>
+#endif
+
+#ifdef QSREF_GPU_TRACKING
+#include "qsg_shim.h"
 #endif
 
 MonteCarlo* mcco = NULL;  // declared extern in Globals.hh
@@ -193,7 +210,14 @@ int main(int argc, char** argv)
 {
     mpiInit(&argc, &argv);
     Parameters params = getParameters(argc, argv);
+    if (getenv("QSREF_PRINT_PARAMS")) {
+        printParameters(params, std::cout);
+        std::cout << "QSREF_PARAMS_END" << std::endl;
+    }
     mcco = initMC(params);
+#ifdef QSREF_GPU_TRACKING
+    if (qsg_shim_attach(mcco, argc, (const char* const*)argv, 0) != 0) return 2;
+#endif
 
     const char* fluxPath = getenv("QSREF_FLUX_DUMP");
     const char* censusPath = getenv("QSREF_CENSUS_DUMP");
@@ -210,7 +234,11 @@ int main(int argc, char** argv)
     {
         cycle_init(mcco, params.simulationParams.loadBalance != 0);
         auto t0 = std::chrono::steady_clock::now();
+#ifdef QSREF_GPU_TRACKING
+        cycleTracking(mcco);
+#else
         cycle_track(mcco);
+#endif
         auto t1 = std::chrono::steady_clock::now();
         double secs = std::chrono::duration<double>(t1 - t0).count();
 
@@ -240,6 +268,12 @@ int main(int argc, char** argv)
     fflush(stdout);
     const char* fluencePath = getenv("QSREF_FLUENCE_DUMP");
     if (fluencePath && params.simulationParams.coralBenchmark) dump_fluence(mcco, fluencePath);
+    // gameOver (main.cc:93): EnergySpectrum::PrintSpectrum, a no-op unless
+    // energySpectrum is set
+    mcco->_tallies->_spectrum.PrintSpectrum(mcco);
+#ifdef QSREF_GPU_TRACKING
+    qsg_shim_detach();
+#endif
     delete mcco;
     mpiFinalize();
     return 0;

@@ -25,16 +25,19 @@ BALANCE_FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fissio
                   "collision", "escape", "census", "num_segments", "end"]
 
 #: every entry point declared in include/qsg.h
-EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_destroy", "qsg_get_info",
-           "qsg_set_tuning", "qsg_get_tuning", "qsg_cycle_init", "qsg_cycle_tracking",
+EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_create_from_deck_opts",
+           "qsg_problem_error", "qsg_destroy", "qsg_get_info",
+           "qsg_set_tuning", "qsg_get_tuning", "qsg_cycle_init", "qsg_cycle_begin_host", "qsg_cycle_tracking",
            "qsg_cycle_finalize", "qsg_run", "qsg_get_scalar_flux", "qsg_set_particles",
            "qsg_get_census_count", "qsg_get_census", "qsg_get_processing_count",
            "qsg_get_processing", "qsg_math_probe", "qsg_get_stats", "qsg_reset_stats",
-           "qsg_batch_create", "qsg_batch_destroy", "qsg_batch_step", "qsg_batch_set_active",
+           "qsg_batch_create", "qsg_batch_destroy", "qsg_batch_step", "qsg_batch_problem_error",
+           "qsg_batch_set_active",
            "qsg_batch_set_tuning", "qsg_batch_get_stats", "qsg_server_create", "qsg_server_destroy",
-           "qsg_server_submit", "qsg_server_step", "qsg_server_status", "qsg_server_result",
-           "qsg_server_release", "qsg_server_set_tuning", "qsg_server_get_stats",
-           "qsg_coral_check", "qsg_get_fluence", "qsg_nearest_facet_probe"]
+           "qsg_server_submit", "qsg_server_submit_opts", "qsg_server_step", "qsg_server_status",
+           "qsg_server_result", "qsg_server_cancel", "qsg_server_release", "qsg_server_set_tuning",
+           "qsg_server_get_stats",
+           "qsg_coral_check", "qsg_get_fluence", "qsg_get_energy_spectrum", "qsg_nearest_facet_probe"]
 
 
 class QsgError(RuntimeError):
@@ -84,6 +87,11 @@ class CoralReport(ctypes.Structure):
             "PASS:: Fluence is homogenous across cells with 6% tolerance" if self.fluence_pass
             else "FAIL:: Fluence not homogenous across cells within 6% tolerance",
         ]
+
+
+class Options(ctypes.Structure):
+    """qsg_options: per-problem sizing (0 = the library's default)."""
+    _fields_ = [("particle_capacity", ctypes.c_uint64)]
 
 
 class Stats(ctypes.Structure):
@@ -199,11 +207,14 @@ def lib() -> ctypes.CDLL:
     L.qsg_last_error.restype = c
     L.qsg_create.argtypes = [i, argv_t, i, _P(vp)]
     L.qsg_create_from_deck.argtypes = [c, i, argv_t, i, _P(vp)]
+    L.qsg_create_from_deck_opts.argtypes = [c, i, argv_t, i, _P(Options), _P(vp)]
+    L.qsg_problem_error.argtypes = [vp, _P(i), c, ctypes.c_size_t]
     L.qsg_destroy.argtypes = [vp]
     L.qsg_get_info.argtypes = [vp, _P(Info)]
     L.qsg_set_tuning.argtypes = [vp, _P(Tuning)]
     L.qsg_get_tuning.argtypes = [vp, _P(Tuning)]
     L.qsg_cycle_init.argtypes = [vp]
+    L.qsg_cycle_begin_host.argtypes = [vp, _P(ParticlesC), ctypes.c_uint64]
     L.qsg_cycle_tracking.argtypes = [vp]
     L.qsg_cycle_finalize.argtypes = [vp, _P(Balance), _D]
     L.qsg_run.argtypes = [vp, i, _P(Balance), _D, _D]
@@ -216,11 +227,14 @@ def lib() -> ctypes.CDLL:
     L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
     _I32 = _P(ctypes.c_int32)
     L.qsg_nearest_facet_probe.argtypes = [vp, ctypes.c_uint64, _D, _D, _I32, _D, _I32, _D]
+    L.qsg_get_energy_spectrum.argtypes = [vp, _D, _P(ctypes.c_uint64), ctypes.c_uint64]
+    L.qsg_get_fluence.argtypes = [vp, _D, ctypes.c_uint64]
     L.qsg_get_stats.argtypes = [vp, _P(Stats)]
     L.qsg_reset_stats.argtypes = [vp]
     L.qsg_batch_create.argtypes = [_P(vp), i, _P(vp)]
     L.qsg_batch_destroy.argtypes = [vp]
     L.qsg_batch_step.argtypes = [vp, _P(Balance), _D]
+    L.qsg_batch_problem_error.argtypes = [vp, i, _P(i)]
     L.qsg_batch_set_active.argtypes = [vp, i, i]
     L.qsg_batch_set_tuning.argtypes = [vp, _P(Tuning)]
     L.qsg_batch_get_stats.argtypes = [vp, _P(Stats)]
@@ -228,6 +242,8 @@ def lib() -> ctypes.CDLL:
     L.qsg_server_create.argtypes = [i, u64, i, _P(vp)]
     L.qsg_server_destroy.argtypes = [vp]
     L.qsg_server_submit.argtypes = [vp, c, _P(u64)]
+    L.qsg_server_submit_opts.argtypes = [vp, c, _P(Options), _P(u64)]
+    L.qsg_server_cancel.argtypes = [vp, u64]
     L.qsg_server_step.argtypes = [vp, _P(i), _P(i)]
     L.qsg_server_status.argtypes = [vp, u64, _P(i), _P(i)]
     L.qsg_server_result.argtypes = [vp, u64, _P(Balance), _D, i]
@@ -254,15 +270,21 @@ def _argv(args: Sequence[str]):
 class Problem:
     """One transport problem resident on a GPU (a qsg_problem handle)."""
 
-    def __init__(self, deck_text: Optional[str] = None, args: Sequence[str] = (), device: int = 0):
+    def __init__(self, deck_text: Optional[str] = None, args: Sequence[str] = (), device: int = 0,
+                 particle_capacity: int = 0):
+        """particle_capacity: entries of each particle buffer (0: the
+        library's default, 2 x nParticles + one cycle's source + 65536)."""
         L = lib()
         h = ctypes.c_void_p()
         argv = ["qs", *args]
         if deck_text is None:
+            if particle_capacity:
+                raise ValueError("particle_capacity needs the deck text")
             _check(L.qsg_create(len(argv), _argv(argv), device, ctypes.byref(h)))
         else:
-            _check(L.qsg_create_from_deck(deck_text.encode(), len(argv), _argv(argv), device,
-                                          ctypes.byref(h)))
+            opts = Options(particle_capacity=int(particle_capacity))
+            _check(L.qsg_create_from_deck_opts(deck_text.encode(), len(argv), _argv(argv), device,
+                                               ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
 
     @classmethod
@@ -310,9 +332,22 @@ class Problem:
     def reset_stats(self):
         _check(lib().qsg_reset_stats(self._h))
 
+    def error(self):
+        """(code, message) the problem failed with; code 0 if it did not."""
+        code = ctypes.c_int()
+        msg = ctypes.create_string_buffer(1024)
+        _check(lib().qsg_problem_error(self._h, ctypes.byref(code), msg, len(msg)))
+        return code.value, msg.value.decode()
+
     # ---- per-cycle phases (main.cc:55-66)
     def cycle_init(self):
         _check(lib().qsg_cycle_init(self._h))
+
+    def cycle_begin_host(self, ps: "ParticleSet"):
+        """In place of cycle_init: ``ps`` (from the caller's own source and
+        population control) becomes this cycle's processing population."""
+        v = ps.c_view()
+        _check(lib().qsg_cycle_begin_host(self._h, ctypes.byref(v), len(ps)))
 
     def cycle_tracking(self):
         _check(lib().qsg_cycle_tracking(self._h))
@@ -338,6 +373,15 @@ class Problem:
         out = np.zeros((inf.n_cells, inf.n_groups), dtype=np.float64)
         _check(lib().qsg_get_scalar_flux(self._h, out.ctypes.data_as(_D), out.size))
         return out
+
+    def energy_spectrum(self):
+        """(group energies [G+1], census counts [G+1]) accumulated over the
+        cycles run so far (runs that set energySpectrum)."""
+        n = self.info().n_groups + 1
+        e = np.zeros(n, dtype=np.float64)
+        c = np.zeros(n, dtype=np.uint64)
+        _check(lib().qsg_get_energy_spectrum(self._h, e.ctypes.data_as(_D), c.ctypes.data_as(_P(ctypes.c_uint64)), n))
+        return e, c
 
     def fluence(self) -> np.ndarray:
         """Cumulative per-cell fluence (decks with coralBenchmark set)."""
@@ -419,14 +463,30 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def step(self):
+    def step(self, strict: bool = True):
         """One cycle of every active problem: list of (balance dict, flux sum)
-        (None for inactive problems)."""
+        (zeros for inactive problems).  A problem whose cycle raised a device
+        error flag is taken out of the batch (``problem_error``) and gets
+        None; the others advance.  strict: raise QsgError after such a step
+        (its other results are then lost to the caller)."""
         n = len(self.problems)
         bal = (Balance * n)()
         flux = (ctypes.c_double * n)()
-        _check(lib().qsg_batch_step(self._h, bal, flux))
-        return [(bal[k].as_dict(), flux[k]) for k in range(n)]
+        rc = lib().qsg_batch_step(self._h, bal, flux)
+        if rc != QSG_OK and (strict or rc not in (3, 5)):
+            _check(rc)
+        out = [(bal[k].as_dict(), flux[k]) for k in range(n)]
+        if rc != QSG_OK:
+            for k in range(n):
+                if self.problem_error(k):
+                    out[k] = None
+        return out
+
+    def problem_error(self, index: int) -> int:
+        """The error code slot ``index``'s problem was taken out with (0: none)."""
+        code = ctypes.c_int()
+        _check(lib().qsg_batch_problem_error(self._h, index, ctypes.byref(code)))
+        return code.value
 
     def set_active(self, index: int, active: bool):
         _check(lib().qsg_batch_set_active(self._h, index, 1 if active else 0))
@@ -461,7 +521,7 @@ class Server:
     """Continuous-batching serving loop (qsg_server): requests are admitted by
     HBM budget, stay resident, and advance together one cycle per step()."""
 
-    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+    QUEUED, RUNNING, DONE, FAILED, CANCELLED = 0, 1, 2, 3, 4
 
     def __init__(self, hbm_budget_bytes: int, max_active: int = 0, device: int = 0):
         h = ctypes.c_void_p()
@@ -485,9 +545,12 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
-    def submit(self, deck_text: str) -> int:
+    def submit(self, deck_text: str, particle_capacity: int = 0) -> int:
+        """Queues a request; particle_capacity sizes its particle buffers
+        (0: the library's default)."""
         rid = ctypes.c_uint64()
-        _check(lib().qsg_server_submit(self._h, deck_text.encode(), ctypes.byref(rid)))
+        opts = Options(particle_capacity=int(particle_capacity))
+        _check(lib().qsg_server_submit_opts(self._h, deck_text.encode(), ctypes.byref(opts), ctypes.byref(rid)))
         return rid.value
 
     def step(self):
@@ -506,6 +569,10 @@ class Server:
         flux = (ctypes.c_double * max(n, 1))()
         _check(lib().qsg_server_result(self._h, rid, bal, flux, n))
         return [(bal[k].as_dict(), flux[k]) for k in range(n)]
+
+    def cancel(self, rid: int):
+        """Stops a queued or running request; its completed cycles stay readable."""
+        _check(lib().qsg_server_cancel(self._h, rid))
 
     def release(self, rid: int):
         _check(lib().qsg_server_release(self._h, rid))

@@ -85,6 +85,10 @@ struct DevProblem {
     // tallies
     double* flux;  // [nCells][G]
     double* fluence;  // [nCells] cumulative over cycles (CORAL2 decks only, else null)
+    // census energy spectrum [G+1], cumulative over cycles: census particles
+    // per energy group (bin G: above eMax); only when the run asks for it
+    // (energySpectrum), else null
+    unsigned long long* spectrum;
 };
 
 // Balance counters accumulated on the device (Tallies.hh:36-100 Balance).
@@ -101,7 +105,8 @@ struct PassCounters {
     uint32_t nCensus;       // census buffer fill (cumulative over the cycle)
     uint32_t nPopOut;       // population-control output fill
     uint32_t error;         // bit 0 census overflow, 1 extra overflow, 2 pop
-                            // overflow, 3 runaway history, 4 nOut > 4
+                            // overflow, 3 runaway history, 4 nOut > 4,
+                            // 5 census + source over the processing buffer
     uint32_t extraCount[2]; // secondaries appended by passes (ping-pong)
     uint32_t pad[2];
 };

@@ -965,6 +965,7 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.lowWeightCutoff = rfld(c.lowWeightCutoff);
     P.flux = rflp(c.flux);
     P.fluence = rflp(c.fluence);
+    P.spectrum = rflp(c.spectrum);
     return P;
 }
 
@@ -1425,6 +1426,10 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             uint32_t o = wave_append(io.nCensus, 1u);
             if (o < io.censusCap) store_particle(io.census, o, L, cell);
             else atomicOr(io.error, 1u);
+            // EnergySpectrum::UpdateSpectrum (EnergySpectrum.cc:12-35, called
+            // from Tallies.cc:97 on every census particle of every cycle):
+            // the particle's getEnergyGroup, which is G above eMax
+            if (P.spectrum) atomicAdd(&P.spectrum[(L.E > T.energies[G]) ? G : L.group], 1ull);
             ev |= kEvCensus;
             active = false;
         }
@@ -2173,7 +2178,7 @@ __global__ void k_batch_prep(BatchSlot* slots, int n, uint64_t* srcPrefix, uint6
         st->nPop = 0;
         st->extra[0] = 0;
         st->extra[1] = 0;
-        st->error = (act && !fits) ? 4u : 0u;
+        st->error = (act && !fits) ? 32u : 0u;
         if (act) {
             st->nCensus = 0;
             unsigned long long* c = (unsigned long long*)dc;

@@ -4,8 +4,10 @@
 //
 // Output: the parameters as a re-runnable deck, one line per cycle in the
 // layout of Tallies::PrintSummary (Tallies.cc:123-144) with the scalar flux
-// printed at full precision (%.17e), and the figure of merit of
-// MC_Fast_Timer.cc:97-104 (segments / tracking seconds).
+// printed at full precision (%.17e), the figure of merit of
+// MC_Fast_Timer.cc:97-104 (segments / tracking seconds), the CORAL2 verdicts
+// for CORAL2 decks, and the files the reference writes: <crossSectionsOut>.dat
+// (-S, at set-up) and <energySpectrum>.dat (-e, at the end).
 #include <cinttypes>
 #include <cstdio>
 #include <iostream>
@@ -16,8 +18,9 @@
 #include "qsg.h"
 
 int main(int argc, char** argv) {
+    qs::Parameters params;
     try {
-        qs::Parameters params = qs::getParameters(argc, argv);
+        params = qs::getParameters(argc, argv);
         qs::printParameters(params, std::cout);
     } catch (const std::exception& e) {
         fprintf(stderr, "qs: %s\n", e.what());
@@ -56,6 +59,27 @@ int main(int argc, char** argv) {
     }
     printf("Figure Of Merit  %14.2e [Num Segments / Cycle Tracking Time]  (%" PRIu64 " segments, %.6e s)\n",
            totalTrack > 0 ? (double)totalSeg / totalTrack : 0.0, totalSeg, totalTrack);
+    // gameOver (main.cc:87-94): the census energy spectrum
+    // (EnergySpectrum::PrintSpectrum, EnergySpectrum.cc:37-58)
+    if (!params.simulationParams.energySpectrum.empty()) {
+        const uint64_t n = (uint64_t)info.n_groups + 1;
+        std::vector<double> e(n);
+        std::vector<uint64_t> counts(n);
+        if (qsg_get_energy_spectrum(p, e.data(), counts.data(), n) != QSG_OK) {
+            fprintf(stderr, "qs: %s\n", qsg_last_error());
+            qsg_destroy(p);
+            return 1;
+        }
+        const std::string name = params.simulationParams.energySpectrum + ".dat";
+        FILE* f = fopen(name.c_str(), "w");
+        if (!f) {
+            fprintf(stderr, "qs: cannot write %s\n", name.c_str());
+            qsg_destroy(p);
+            return 1;
+        }
+        for (uint64_t g = 0; g < n; g++) fprintf(f, "%d\t%g\t%" PRIu64 "\n", (int)g, e[g], counts[g]);
+        fclose(f);
+    }
     // coralBenchmarkCorrectness (main.cc:73): the CORAL2 checks, printed in
     // the reference's words
     qsg_coral_report r;

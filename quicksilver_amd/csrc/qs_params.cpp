@@ -31,6 +31,18 @@ struct Block {
         if (!in) throw std::runtime_error("cannot parse value of '" + std::string(key) +
                                           "' in block " + name + ": '" + it->second + "'");
     }
+    // A string key given with no value ("energySpectrum: " and
+    // "crossSectionsOut:", as printParameters writes unset names,
+    // Parameters.cc:124,151) stays empty: the reference's extraction fails,
+    // its qs_assert only prints (InputBlock.hh:36-52), and the name is left
+    // empty, so a printed deck runs again.
+    void get(const char* key, std::string& value) const {
+        auto it = kv.find(key);
+        if (it == kv.end()) return;
+        std::istringstream in(it->second);
+        value.clear();
+        in >> value;
+    }
 };
 
 void trim(std::string& s) {

@@ -11,6 +11,7 @@
 //             (the reference's extra vaults, ParticleVaultContainer.cc:341-400)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -80,6 +81,21 @@ void copyParticles(const qs::Particles& d, const qsg_particles* h, uint64_t n, h
 
 void setDevice(qsg_problem_s* p) { hipCheck(hipSetDevice(p->device), "hipSetDevice"); }
 
+// Runs device work of a cycle; if it throws, the problem's device state is
+// no longer a cycle boundary, so the problem is marked failed first.
+template <class F>
+void deviceStage(qsg_problem_s* p, F work) {
+    try {
+        work();
+    } catch (const QsgError& e) {
+        qsgi::markFailed(p, e.code, e.what());
+        throw;
+    } catch (const std::exception& e) {
+        qsgi::markFailed(p, QSG_ERR_DEVICE, e.what());
+        throw;
+    }
+}
+
 // Events that only time launches: recording one without the system-scope
 // fence (cache write-back and invalidation) leaves no gap of several
 // microseconds in the stream.  Nothing that the host reads depends on them:
@@ -100,20 +116,11 @@ void syncCounters(qsg_problem_s* p) {
 }
 
 void checkErrors(uint32_t e) {
-    if (!e) return;
-    std::string m = "device error flags:";
-    if (e & 1u) m += " census buffer overflow;";
-    if (e & 2u) m += " secondary buffer overflow;";
-    if (e & 4u) m += " population-control buffer overflow;";
-    if (e & 8u) m += " runaway particle history;";
-    if (e & 16u) m += " fission produced more than 4 particles;";
-    if (e & 32u) m += " processing population exceeds capacity;";
-    throw QsgError((e & 39u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC,
-                   m + " (raise QSG_CAPACITY_FACTOR for capacity errors)");
+    if (e) throw QsgError(qsgi::deviceErrorCode(e), qsgi::deviceErrorText(e));
 }
 void checkErrors(qsg_problem_s* p) { checkErrors(p->hpc->error); }
 
-void build(qsg_problem_s* p, int device) {
+void build(qsg_problem_s* p, int device, uint64_t capacity) {
     p->device = device;
     setDevice(p);
     p->stream = qsgi::acquireStream(device);
@@ -202,6 +209,11 @@ void build(qsg_problem_s* p, int device) {
     hipCheck(hipMemset(d.flux, 0, (size_t)T.nCells * T.nGroups * sizeof(double)), "hipMemset");
     p->sortCounts = p->dalloc<uint32_t>(qs::sortKeys(T.nCells));
     p->sortBlockSums = p->dalloc<uint32_t>(qs::scanBlocks(qs::sortKeys(T.nCells)));
+    d.spectrum = nullptr;
+    if (!p->params.simulationParams.energySpectrum.empty()) {
+        d.spectrum = p->dalloc<unsigned long long>((size_t)T.nGroups + 1);
+        hipCheck(hipMemset(d.spectrum, 0, ((size_t)T.nGroups + 1) * sizeof(unsigned long long)), "hipMemset");
+    }
     d.fluence = nullptr;
     if (p->params.simulationParams.coralBenchmark) {
         d.fluence = p->dalloc<double>((size_t)T.nCells);
@@ -209,7 +221,7 @@ void build(qsg_problem_s* p, int device) {
     }
 
     // particle buffers
-    uint32_t cap = qsgi::particleCapacity(T);
+    uint32_t cap = qsgi::particleCapacity(T, capacity);
     p->proc[0] = makeParticles(p, cap);
     p->proc[1] = makeParticles(p, cap);
     p->pop = makeParticles(p, cap);
@@ -361,11 +373,13 @@ void closeCycle(qsg_problem_s* p, const qs::DevCounters& d, uint32_t nCensus, qs
     // particle loads: every population-control survivor plus every
     // secondary record (start + source + split - rr + produce when each
     // fission emits >= 2 particles)
-    p->stats.particle_loads += p->startCount + T.sourcePerCycle + d.split - d.rr + d.produce;
+    const uint64_t start = p->hostCycle ? p->hostCount : p->startCount;
+    const uint64_t source = p->hostCycle ? 0 : T.sourcePerCycle;
+    p->stats.particle_loads += start + source + d.split - d.rr + d.produce;
     p->stats.cycles += 1;
     qsg_balance b;
-    b.start = p->startCount;
-    b.source = T.sourcePerCycle;
+    b.start = start;
+    b.source = source;
     b.rr = d.rr;
     b.split = d.split;
     b.absorb = d.absorb;
@@ -385,38 +399,79 @@ void closeCycle(qsg_problem_s* p, const qs::DevCounters& d, uint32_t nCensus, qs
     p->cur = p->censusBuf;
     p->cycle++;
     p->phase = 0;
+    p->hostCycle = false;
 }
 
 // ---- the three per-cycle entry points: each synchronises once it is done
 
 void cycleInit(qsg_problem_s* p) {
+    qsgi::requireUsable(p);
     if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_init called twice without finalize");
     setDevice(p);
     if (p->startCount + p->tables.sourcePerCycle > p->proc[p->cur].cap)
         throw QsgError(QSG_ERR_CAPACITY, "processing population exceeds capacity");
-    hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
-    p->queueSorted = enqueueSetUp(p, p->cycle, p->cur, true, false);
-    hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
-    syncCounters(p);
-    checkErrors(p);
+    deviceStage(p, [&] {
+        hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
+        p->queueSorted = enqueueSetUp(p, p->cycle, p->cur, true, false);
+        hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
+        syncCounters(p);
+        checkErrors(p);
+    });
     p->nPop = p->hpc->nPopOut;
     p->stats.init_seconds += elapsed(p->evI0, p->evI1);
     p->phase = 1;
 }
 
+// A cycle whose processing population is the caller's: the caller ran the
+// reference's own cycleInit (source, population control, low-weight
+// roulette, main.cc:96-121) on the host, and the given particles are the
+// queue of the first tracking pass.  The counters are cleared and nothing
+// else runs on the device.
+void cycleBeginHost(qsg_problem_s* p, const qsg_particles* in, uint64_t n) {
+    qsgi::requireUsable(p);
+    if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "cycle_begin_host needs a problem at a cycle boundary");
+    if (n > p->pop.cap) throw QsgError(QSG_ERR_CAPACITY, "more particles than the particle capacity");
+    if (n && !in) throw QsgError(QSG_ERR_ARG, "NULL particles");
+    for (uint64_t i = 0; i < n; i++)
+        if (in->cell[i] < 0 || in->cell[i] >= p->tables.nCells)
+            throw QsgError(QSG_ERR_ARG, "particle cell id out of range");
+    setDevice(p);
+    const qs::ProblemTables& T = p->tables;
+    deviceStage(p, [&] {
+        hipCheck(hipEventRecord(p->evI0, p->stream), "hipEventRecord");
+        // zero carried and zero sourced: the set-up clears every counter
+        qs::launchCycleSetup(p->ctx(), p->cb, 0u, p->proc[p->cur].cap, 0, T.nParticles, T.loadBalance);
+        copyParticles(p->pop.v, in, n, hipMemcpyHostToDevice, p->stream);
+        const uint32_t n32 = (uint32_t)n;
+        hipCheck(hipMemcpyAsync(&p->pc->nPopOut, &n32, sizeof(uint32_t), hipMemcpyHostToDevice, p->stream),
+                 "hipMemcpyAsync");
+        hipCheck(hipEventRecord(p->evI1, p->stream), "hipEventRecord");
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        p->stats.init_seconds += elapsed(p->evI0, p->evI1);
+    });
+    p->nPop = (uint32_t)n;
+    p->queueSorted = false;
+    p->hostCycle = true;
+    p->hostCount = n;
+    p->phase = 1;
+}
+
 void cycleTracking(qsg_problem_s* p) {
+    qsgi::requireUsable(p);
     if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "cycle_tracking called before cycle_init");
     setDevice(p);
     const int cur = p->cur;
-    hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
-    const int passes = runPasses(p, 0, cur, p->queueSorted ? p->proc[cur].v : p->pop.v, &p->pc->nPopOut, p->nPop,
-                                 p->ev1);
-    if (passes == 0) {
-        // nothing to track: the counters are those the set-up left
-        hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
-        hipCheck(hipEventSynchronize(p->ev1), "hipEventSynchronize");
-    }
-    p->lastTrackingSeconds = elapsed(p->ev0, p->ev1);
+    deviceStage(p, [&] {
+        hipCheck(hipEventRecord(p->ev0, p->stream), "hipEventRecord");
+        const int passes = runPasses(p, 0, cur, p->queueSorted ? p->proc[cur].v : p->pop.v, &p->pc->nPopOut,
+                                     p->nPop, p->ev1);
+        if (passes == 0) {
+            // nothing to track: the counters are those the set-up left
+            hipCheck(hipEventRecord(p->ev1, p->stream), "hipEventRecord");
+            hipCheck(hipEventSynchronize(p->ev1), "hipEventSynchronize");
+        }
+        p->lastTrackingSeconds = elapsed(p->ev0, p->ev1);
+    });
     p->stats.tracking_seconds += p->lastTrackingSeconds;
     p->nCensus = p->hpc->nCensus;
     p->censusBuf = 1 - cur;
@@ -424,13 +479,16 @@ void cycleTracking(qsg_problem_s* p) {
 }
 
 void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
+    qsgi::requireUsable(p);
     if (p->phase != 2) throw QsgError(QSG_ERR_STATE, "cycle_finalize called before cycle_tracking");
     setDevice(p);
-    hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
-    enqueueFinalize(p);
-    hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
-    hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
-    p->stats.finalize_seconds += elapsed(p->evF0, p->evF1);
+    deviceStage(p, [&] {
+        hipCheck(hipEventRecord(p->evF0, p->stream), "hipEventRecord");
+        enqueueFinalize(p);
+        hipCheck(hipEventRecord(p->evF1, p->stream), "hipEventRecord");
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        p->stats.finalize_seconds += elapsed(p->evF0, p->evF1);
+    });
     // the tallies and the flux sum came back with the tracking's last
     // synchronisation, and no finalize kernel changes them
     closeCycle(p, *p->hdc, p->nCensus, bal, fluxSum);
@@ -450,6 +508,7 @@ void cycleFinalize(qsg_problem_s* p, qsg_balance* bal, double* fluxSum) {
 // each first pass only; the device time between two first passes (copy,
 // finalize, set-up) is counted as set-up, the last cycle's as finalize.
 void runCycles(qsg_problem_s* p, int nCycles, qsg_balance* bals, double* fluxSums, double* trackSec) {
+    qsgi::requireUsable(p);
     if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "qsg_run needs a problem at a cycle boundary");
     if (nCycles <= 0) return;
     setDevice(p);
@@ -522,15 +581,62 @@ void runCycles(qsg_problem_s* p, int nCycles, qsg_balance* bals, double* fluxSum
             prevEnd = end;
         }
         p->stats.finalize_seconds += elapsed(prevEnd, p->runEnd);
-    } catch (...) {
+    } catch (const QsgError& e) {
         // nothing still in flight may write into the caller's arrays later
         (void)hipStreamSynchronize(p->stream);
+        qsgi::markFailed(p, e.code, e.what());
+        throw;
+    } catch (const std::exception& e) {
+        (void)hipStreamSynchronize(p->stream);
+        qsgi::markFailed(p, QSG_ERR_DEVICE, e.what());
         throw;
     }
 }
 
-qsg_problem_s* createFromParams(const qs::Parameters& params, int device) {
-    return qsgi::createFromTables(params, qs::buildProblem(params), device, true);
+// checkCrossSections (initMC.cc:392-484, called by initMC at initMC.cc:72):
+// with crossSectionsOut set, <name>.dat lists per group the group's mid
+// energy and, per material in name order, the absorption, fission and
+// scatter cross sections summed over the material's isotopes and reactions
+// of each type, each term divided by the isotope count, in the reference's
+// loop order (isotope, then reaction).
+void writeCrossSections(const qs::ProblemTables& T, const std::string& name) {
+    const int G = T.nGroups;
+    std::vector<const qs::MaterialTables*> mats;
+    for (const qs::MaterialTables& m : T.materials) mats.push_back(&m);
+    std::sort(mats.begin(), mats.end(), [](const qs::MaterialTables* a, const qs::MaterialTables* b) {
+        return a->name < b->name;
+    });
+    FILE* f = fopen((name + ".dat").c_str(), "w");
+    if (!f) throw QsgError(QSG_ERR_ARG, "cannot write " + name + ".dat");
+    fprintf(f, "#group  energy");
+    for (const qs::MaterialTables* m : mats)
+        fprintf(f, "  %s_a  %s_f  %s_s", m->name.c_str(), m->name.c_str(), m->name.c_str());
+    fprintf(f, "\n");
+    std::vector<double> a(mats.size() * (size_t)G, 0.0), fi(a.size(), 0.0), sc(a.size(), 0.0);
+    for (size_t k = 0; k < mats.size(); k++) {
+        const qs::MaterialTables& m = *mats[k];
+        for (int iso = 0; iso < m.nIsotopes; iso++)
+            for (int r = 0; r < m.nReactions; r++)
+                for (int g = 0; g < G; g++) {
+                    const double v = m.micro[(size_t)r * G + g] / m.nIsotopes;
+                    double* dst = m.reactionType[r] == qs::kScatter ? &sc[k * G + g]
+                                  : m.reactionType[r] == qs::kAbsorption ? &a[k * G + g]
+                                                                         : &fi[k * G + g];
+                    *dst += v;
+                }
+    }
+    for (int g = 0; g < G; g++) {
+        fprintf(f, "%u  %g", (unsigned)g, (T.energies[g] + T.energies[g + 1]) / 2.0);
+        for (size_t k = 0; k < mats.size(); k++) fprintf(f, "  %g  %g  %g", a[k * G + g], fi[k * G + g], sc[k * G + g]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+}
+
+qsg_problem_s* createFromParams(const qs::Parameters& params, int device, uint64_t capacity = 0) {
+    qs::ProblemTables T = qs::buildProblem(params);
+    if (!params.simulationParams.crossSectionsOut.empty()) writeCrossSections(T, params.simulationParams.crossSectionsOut);
+    return qsgi::createFromTables(params, std::move(T), device, true, capacity);
 }
 
 }  // namespace
@@ -630,33 +736,55 @@ void qsgi::tuningToC(const qs::TrackTuning& in, qsg_tuning* t) {
     t->graphs = in.graphs;
 }
 
-uint32_t qsgi::particleCapacity(const qs::ProblemTables& T) {
-    double factor = 2.0;
-    if (const char* env = getenv("QSG_CAPACITY_FACTOR")) factor = atof(env);
-    if (!(factor >= 1.0)) factor = 2.0;
-    double capD = factor * (double)T.nParticles + (double)T.sourcePerCycle + 65536.0;
+uint32_t qsgi::particleCapacity(const qs::ProblemTables& T, uint64_t requested) {
+    const double capD = requested ? (double)requested
+                                  : 2.0 * (double)T.nParticles + (double)T.sourcePerCycle + 65536.0;
     if (capD > 4.0e9) throw QsgError(QSG_ERR_CAPACITY, "particle capacity exceeds 32-bit indexing");
     return (uint32_t)capD;
 }
 
-uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T) {
+std::string qsgi::deviceErrorText(uint32_t e) {
+    std::string m = "device error flags:";
+    if (e & 1u) m += " census buffer overflow;";
+    if (e & 2u) m += " secondary buffer overflow;";
+    if (e & 4u) m += " population-control buffer overflow;";
+    if (e & 8u) m += " runaway particle history;";
+    if (e & 16u) m += " fission produced more than 4 particles;";
+    if (e & 32u) m += " census + source exceed the processing buffer;";
+    if (e & (1u | 2u | 4u | 32u)) m += " (a larger qsg_options.particle_capacity avoids capacity errors)";
+    return m;
+}
+
+int qsgi::deviceErrorCode(uint32_t e) { return (e & (1u | 2u | 4u | 32u)) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC; }
+
+void qsgi::markFailed(qsg_problem_s* p, int code, const std::string& msg) {
+    if (!p || p->failCode) return;
+    p->failCode = code ? code : QSG_ERR_DEVICE;
+    p->failMsg = msg;
+}
+
+void qsgi::requireUsable(const qsg_problem_s* p) {
+    if (p->failCode) throw QsgError(QSG_ERR_STATE, "the problem failed earlier (" + p->failMsg + ")");
+}
+
+uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T, uint64_t requested) {
     const uint64_t nc = (uint64_t)T.nCells, G = (uint64_t)T.nGroups, M = T.materials.size();
     const uint64_t R = (uint64_t)T.maxReactions;
     uint64_t b = nc * 96 * 8 + nc * 4 + nc * 8 + nc * 24 + (nc + 1) * 8 + nc * G * 8;
     b += (G + 1) * 8 + 2 * M * G * 8 + M * R * G * 8 + M * R * 4;
-    const uint64_t cap = particleCapacity(T);
+    const uint64_t cap = particleCapacity(T, requested);
     b += 5 * (cap * qs::kParticleBytes + 12 * 256) + qs::popGroups((uint32_t)cap) * sizeof(qs::PopGroup);
     b += nc * 24 * 16 + qs::sortKeys((int)nc) * 4 + qs::scanBlocks(qs::sortKeys((int)nc)) * 4;
     return b + 64 * 1024;
 }
 
 qsg_problem_s* qsgi::createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device,
-                                      bool ownStacks) {
+                                      bool ownStacks, uint64_t capacity) {
     qsg_problem_s* p = new qsg_problem_s();
     try {
         p->params = params;
         p->tables = std::move(tables);
-        build(p, device);
+        build(p, device, capacity);
         // Wave stacks for this problem's own tracking launches, sized for the
         // largest grid its population can fill (allocated now so that running
         // the problem allocates nothing unless its launch shape grows).
@@ -689,6 +817,11 @@ int qsg_create(int argc, const char* const* argv, int device, qsg_problem* out) 
 
 int qsg_create_from_deck(const char* deck_text, int argc, const char* const* argv, int device,
                          qsg_problem* out) {
+    return qsg_create_from_deck_opts(deck_text, argc, argv, device, nullptr, out);
+}
+
+int qsg_create_from_deck_opts(const char* deck_text, int argc, const char* const* argv, int device,
+                              const qsg_options* opts, qsg_problem* out) {
     QSG_GUARD({
         if (!out || !deck_text) throw QsgError(QSG_ERR_ARG, "NULL argument");
         *out = nullptr;
@@ -700,12 +833,24 @@ int qsg_create_from_deck(const char* deck_text, int argc, const char* const* arg
         params.crossSectionParams.clear();
         qs::parseDeckText(deck_text, params);
         qs::supplyDefaults(params);
-        *out = createFromParams(params, device);
+        *out = createFromParams(params, device, opts ? opts->particle_capacity : 0);
     })
 }
 
 int qsg_destroy(qsg_problem p) {
     QSG_GUARD({ destroyProblem(p); })
+}
+
+int qsg_problem_error(qsg_problem p, int* code, char* message, size_t message_len) {
+    QSG_GUARD({
+        if (!p || !code) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        *code = p->failCode;
+        if (message && message_len) {
+            const size_t n = std::min(p->failMsg.size(), message_len - 1);
+            memcpy(message, p->failMsg.data(), n);
+            message[n] = 0;
+        }
+    })
 }
 
 int qsg_get_info(qsg_problem p, qsg_info* info) {
@@ -748,6 +893,13 @@ int qsg_cycle_init(qsg_problem p) {
     QSG_GUARD({
         if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
         cycleInit(p);
+    })
+}
+
+int qsg_cycle_begin_host(qsg_problem p, const qsg_particles* in, uint64_t n) {
+    QSG_GUARD({
+        if (!p) throw QsgError(QSG_ERR_ARG, "NULL problem");
+        cycleBeginHost(p, in, n);
     })
 }
 
@@ -801,6 +953,7 @@ int qsg_get_scalar_flux(qsg_problem p, double* out, uint64_t n) {
 int qsg_set_particles(qsg_problem p, const qsg_particles* in, uint64_t n) {
     QSG_GUARD({
         if (!p || !in) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        qsgi::requireUsable(p);
         if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "set_particles must follow cycle_init");
         if (n > p->pop.cap) throw QsgError(QSG_ERR_CAPACITY, "too many particles");
         for (uint64_t i = 0; i < n; i++)
@@ -825,6 +978,7 @@ int qsg_get_census_count(qsg_problem p, uint64_t* n) {
 int qsg_get_census(qsg_problem p, qsg_particles* out, uint64_t n) {
     QSG_GUARD({
         if (!p || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        qsgi::requireUsable(p);
         if (p->censusBuf < 0) throw QsgError(QSG_ERR_STATE, "no census yet");
         if (n > p->nCensus) throw QsgError(QSG_ERR_ARG, "n exceeds the census count");
         setDevice(p);
@@ -842,6 +996,7 @@ int qsg_get_processing_count(qsg_problem p, uint64_t* n) {
 int qsg_get_processing(qsg_problem p, qsg_particles* out, uint64_t n) {
     QSG_GUARD({
         if (!p || !out) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        qsgi::requireUsable(p);
         if (p->phase != 1) throw QsgError(QSG_ERR_STATE, "processing population exists after cycle_init");
         if (n > p->nPop) throw QsgError(QSG_ERR_ARG, "n exceeds the processing count");
         setDevice(p);
@@ -857,6 +1012,20 @@ int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells) {
         setDevice(p);
         hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
         hipCheck(hipMemcpy(fluence, p->dp.fluence, n_cells * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    })
+}
+
+int qsg_get_energy_spectrum(qsg_problem p, double* energies, uint64_t* counts, uint64_t n) {
+    QSG_GUARD({
+        if (!p || !counts) throw QsgError(QSG_ERR_ARG, "NULL argument");
+        if (!p->dp.spectrum)
+            throw QsgError(QSG_ERR_STATE, "the census spectrum is kept only for runs that set energySpectrum");
+        if (n != (uint64_t)p->tables.nGroups + 1) throw QsgError(QSG_ERR_ARG, "n must be n_groups + 1");
+        setDevice(p);
+        hipCheck(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+        hipCheck(hipMemcpy(counts, p->dp.spectrum, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (energies)
+            for (uint64_t g = 0; g < n; g++) energies[g] = p->tables.energies[g];
     })
 }
 

@@ -35,15 +35,20 @@ struct qsg_batch_s {
     double* hFluxSums = nullptr;
     std::vector<uint32_t> active;
     std::vector<int> expectedCycle;  // each problem's cycle as the batch left it
+    std::vector<int> slotError;      // code a slot's problem was taken out with (0: none)
+    // set when a HIP failure ends a step: the members' device state is then
+    // unknown, so no further step runs
+    bool broken = false;
+    std::string brokenMsg;
     qs::TrackTuning tune = qs::kDefaultTuning;
     qsgi::StackPool ownStacks;
     qsgi::StackPool* stacks = &ownStacks;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> passEv;
-    // captured launch sequences of a step (first half up to the first
-    // tracking pass, second half from the flux reduction on) and what they
-    // were captured with
-    hipGraphExec_t gInit = nullptr, gFinal = nullptr;
+    // captured launch sequences of a step (set-up up to the first tracking
+    // pass, that pass, and the flux reduction on) and what they were
+    // captured with
+    hipGraphExec_t gSetup = nullptr, gTrack = nullptr, gFinal = nullptr;
     qs::TrackTuning graphTune = {};
     qs::WaveStacks graphStacks = {};
     qsg_stats stats = {};
@@ -78,7 +83,7 @@ void destroyBatch(qsg_batch_s* b) {
     for (hipEvent_t e : {b->ev0, b->ev1})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->passEv) (void)hipEventDestroy(e);
-    for (hipGraphExec_t g : {b->gInit, b->gFinal})
+    for (hipGraphExec_t g : {b->gSetup, b->gTrack, b->gFinal})
         if (g) (void)hipGraphExecDestroy(g);
     b->ownStacks.release();
     if (b->stream) {
@@ -89,6 +94,7 @@ void destroyBatch(qsg_batch_s* b) {
 }
 
 void checkMember(const qsg_batch_s* b, const qsg_problem_s* p) {
+    qsgi::requireUsable(p);
     if (p->device != b->device) throw QsgError(QSG_ERR_ARG, "batch problems must share one device");
     if (p->phase != 0) throw QsgError(QSG_ERR_STATE, "batch problems must be at a cycle boundary");
     if (p->cur != (p->cycle & 1)) throw QsgError(QSG_ERR_STATE, "problem buffer parity does not match its cycle");
@@ -152,6 +158,7 @@ void setMember(qsg_batch_s* b, int i, qsg_problem_s* p) {
     b->probs[i] = p;
     b->active[i] = p ? 1u : 0u;
     b->expectedCycle[i] = p ? p->cycle : 0;
+    b->slotError[i] = 0;
     hipCheck(hipMemcpyAsync(b->dSlots + i, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
     hipCheck(hipMemcpyAsync(b->dState + i, &st, sizeof(st), hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
@@ -183,6 +190,7 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, int capacity, int device,
         b->probs.assign(c, nullptr);
         b->active.assign(c, 0);
         b->expectedCycle.assign(c, 0);
+        b->slotError.assign(c, 0);
         b->dSlots = devAlloc<qs::BatchSlot>(c);
         b->dState = devAlloc<qs::BatchState>(c);
         b->dCounters = devAlloc<qs::DevCounters>(c);
@@ -211,18 +219,35 @@ void readStates(qsg_batch_s* b) {
     hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
 }
 
-void checkBatchErrors(qsg_batch_s* b) {
+// Problems whose device error flags are set (in the host copy of the
+// states) are taken out of the batch: marked failed, made inactive on the
+// device so that no later launch of this or any step touches them, and
+// left out of the rest of the step.  Every other problem continues; the
+// device kernels keep each problem's stores inside its own buffers, so a
+// problem's overflow does not reach the others.  Returns how many were
+// taken out; the first one's code and every message go to code / msg.
+int takeOutFlagged(qsg_batch_s* b, int& code, std::string& msg) {
+    static const uint32_t kInactive = 0u;
+    int taken = 0;
     for (size_t i = 0; i < b->probs.size(); i++) {
-        uint32_t e = b->hState[i].error;
-        if (!e) continue;
-        std::string m = "problem " + std::to_string(i) + ": device error flags:";
-        if (e & 1u) m += " census buffer overflow;";
-        if (e & 2u) m += " secondary buffer overflow;";
-        if (e & 4u) m += " population-control buffer overflow;";
-        if (e & 8u) m += " runaway particle history;";
-        if (e & 16u) m += " fission produced more than 4 particles;";
-        throw QsgError((e & 7u) ? QSG_ERR_CAPACITY : QSG_ERR_NUMERIC, m);
+        qs::BatchState& st = b->hState[i];
+        qsg_problem_s* p = b->probs[i];
+        if (!p || !st.active || !st.error) continue;
+        const int c = qsgi::deviceErrorCode(st.error);
+        const std::string m = "batch problem " + std::to_string(i) + ": " + qsgi::deviceErrorText(st.error);
+        qsgi::markFailed(p, c, m);
+        b->slotError[i] = c;
+        b->active[i] = 0;
+        st.active = 0;
+        hipCheck(hipMemcpyAsync(&b->dState[i].active, &kInactive, sizeof(uint32_t), hipMemcpyHostToDevice,
+                                b->stream),
+                 "hipMemcpyAsync");
+        if (!code) code = c;
+        msg += (msg.empty() ? "" : " ") + m;
+        taken++;
     }
+    if (taken) hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return taken;
 }
 
 hipEvent_t passEvent(qsg_batch_s* b, int i) {
@@ -234,12 +259,9 @@ hipEvent_t passEvent(qsg_batch_s* b, int i) {
     return b->passEv[i];
 }
 
-// First half of a step: cycle set-up, source and population control of
-// every problem, the first tracking pass, and the state copy to the host.
-// timed: bracket the tracking pass with events (eager launches only; a
-// captured graph is timed as a whole from outside, since event records
-// inside a graph do not yield elapsed times).
-void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws, bool timed) {
+// First part of a step: cycle set-up, source and population control of
+// every problem, up to the first tracking pass.
+void enqueueSetup(qsg_batch_s* b, const qs::LaunchCtx& c) {
     const int n = (int)b->probs.size();
     qs::launchBatchPrep(c, b->dSlots, n, b->dSrcPrefix, b->dPopPrefix);
     qs::launchBatchSource(c, b->dSlots, n, b->dSrcPrefix);
@@ -248,12 +270,13 @@ void enqueueInit(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& w
         qs::launchBatchSort(c, b->dSlots, n, b->tune.sortInterval, b->dKeyBase, b->dSortPrefix, b->dSortCounts,
                             b->sortKeys, b->dSortBlockSums);
     qs::launchBatchPassPrep(c, b->dSlots, n, 0);
-    if (timed) hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
-    qs::launchBatchTrack(c, b->dSlots, n, 0, b->tune, ws);
+}
+
+// The first tracking pass.  Captured as a graph of its own, so that events
+// recorded between graph launches time the tracking kernel alone.
+void enqueueFirstPass(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws) {
+    qs::launchBatchTrack(c, b->dSlots, (int)b->probs.size(), 0, b->tune, ws);
     hipCheck(hipGetLastError(), "batch track launch");
-    if (timed) hipCheck(hipEventRecord(b->passEv[1], b->stream), "hipEventRecord");
-    hipCheck(hipMemcpyAsync(b->hState, b->dState, n * sizeof(qs::BatchState), hipMemcpyDeviceToHost, b->stream),
-             "hipMemcpyAsync");
 }
 
 // Second half: flux reduction and cycle advance, and the copies of the
@@ -299,15 +322,16 @@ bool sameShape(const qs::TrackTuning& a, const qs::TrackTuning& t) {
 // The step's two graphs, captured again whenever the launch shape or the
 // stack memory changed since they were captured.
 void ensureGraphs(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws) {
-    if (b->gInit && b->gFinal && b->graphsValid && sameShape(b->graphTune, b->tune) &&
+    if (b->gSetup && b->gTrack && b->gFinal && b->graphsValid && sameShape(b->graphTune, b->tune) &&
         b->graphStacks.base == ws.base && b->graphStacks.cap == ws.cap && b->graphStacks.nWaves == ws.nWaves)
         return;
-    for (hipGraphExec_t* g : {&b->gInit, &b->gFinal})
+    for (hipGraphExec_t* g : {&b->gSetup, &b->gTrack, &b->gFinal})
         if (*g) {
             (void)hipGraphExecDestroy(*g);
             *g = nullptr;
         }
-    b->gInit = capture(b, [&] { enqueueInit(b, c, ws, false); });
+    b->gSetup = capture(b, [&] { enqueueSetup(b, c); });
+    b->gTrack = capture(b, [&] { enqueueFirstPass(b, c, ws); });
     b->gFinal = capture(b, [&] { enqueueFinal(b, c); });
     b->graphTune = b->tune;
     b->graphStacks = ws;
@@ -319,12 +343,36 @@ void ensureGraphs(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& 
 // graph launches and two host synchronisations: the first tracking pass
 // tracks secondaries from the waves' stacks, and further passes (launched
 // one by one) are needed only for secondaries that overflowed a stack.
+void stepMembers(qsg_batch_s* b, qsg_balance* balances, double* fluxSums, int& failCode, std::string& failMsg);
+
 void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
+    if (b->broken) throw QsgError(QSG_ERR_STATE, "the batch failed earlier (" + b->brokenMsg + ")");
     hipCheck(hipSetDevice(b->device), "hipSetDevice");
     const int n = (int)b->probs.size();
     for (int i = 0; i < n; i++)
         if (b->probs[i] && (b->probs[i]->phase != 0 || b->probs[i]->cycle != b->expectedCycle[i]))
             throw QsgError(QSG_ERR_STATE, "a batch problem was advanced outside the batch");
+    int failCode = 0;
+    std::string failMsg;
+    try {
+        stepMembers(b, balances, fluxSums, failCode, failMsg);
+    } catch (const std::exception& e) {
+        // a HIP failure part way through the step: every member's device
+        // state is unknown
+        (void)hipStreamSynchronize(b->stream);
+        b->broken = true;
+        b->brokenMsg = e.what();
+        const QsgError* q = dynamic_cast<const QsgError*>(&e);
+        for (int i = 0; i < n; i++)
+            if (b->probs[i] && b->active[i]) qsgi::markFailed(b->probs[i], q ? q->code : QSG_ERR_DEVICE, e.what());
+        throw;
+    }
+    // problems taken out during the step: the others advanced normally
+    if (failCode) throw QsgError(failCode, failMsg);
+}
+
+void stepMembers(qsg_batch_s* b, qsg_balance* balances, double* fluxSums, int& failCode, std::string& failMsg) {
+    const int n = (int)b->probs.size();
     qs::LaunchCtx c{(void*)b->stream, b->numCUs};
     const long long blocks = qs::batchTrackBlocks(c, b->tune);
     const qs::WaveStacks ws = b->stacks->ensure((uint32_t)(blocks * (b->tune.blockSize / 64)),
@@ -334,16 +382,16 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
     if (graphs) ensureGraphs(b, c, ws);
 
     hipCheck(hipEventRecord(b->ev0, b->stream), "hipEventRecord");
-    if (graphs) {
-        // the whole first half is timed as the first pass
-        hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
-        hipCheck(hipGraphLaunch(b->gInit, b->stream), "hipGraphLaunch");
-        hipCheck(hipEventRecord(b->passEv[1], b->stream), "hipEventRecord");
-    } else {
-        enqueueInit(b, c, ws, true);
-    }
+    if (graphs) hipCheck(hipGraphLaunch(b->gSetup, b->stream), "hipGraphLaunch");
+    else enqueueSetup(b, c);
+    hipCheck(hipEventRecord(b->passEv[0], b->stream), "hipEventRecord");
+    if (graphs) hipCheck(hipGraphLaunch(b->gTrack, b->stream), "hipGraphLaunch");
+    else enqueueFirstPass(b, c, ws);
+    hipCheck(hipEventRecord(b->passEv[1], b->stream), "hipEventRecord");
+    hipCheck(hipMemcpyAsync(b->hState, b->dState, n * sizeof(qs::BatchState), hipMemcpyDeviceToHost, b->stream),
+             "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-    checkBatchErrors(b);
+    takeOutFlagged(b, failCode, failMsg);
     // further passes for secondaries that overflowed the stacks
     int k = 1;
     for (;;) {
@@ -360,7 +408,7 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
         hipCheck(hipGetLastError(), "batch track launch");
         hipCheck(hipEventRecord(e1, b->stream), "hipEventRecord");
         readStates(b);
-        checkBatchErrors(b);
+        takeOutFlagged(b, failCode, failMsg);
         k++;
     }
     double trackSeconds = 0;
@@ -373,7 +421,6 @@ void batchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) {
     else enqueueFinal(b, c);
     hipCheck(hipEventRecord(b->ev1, b->stream), "hipEventRecord");
     hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-    checkBatchErrors(b);
     float stepMs = 0;
     hipCheck(hipEventElapsedTime(&stepMs, b->ev0, b->ev1), "hipEventElapsedTime");
 
@@ -438,6 +485,10 @@ void qsgiBatchSetMember(qsg_batch_s* b, int slot, qsg_problem_s* p) {
     setMember(b, slot, p);
 }
 int qsgiBatchCapacity(const qsg_batch_s* b) { return (int)b->probs.size(); }
+int qsgiBatchSlotError(const qsg_batch_s* b, int slot) {
+    return (slot >= 0 && slot < (int)b->slotError.size()) ? b->slotError[slot] : 0;
+}
+bool qsgiBatchBroken(const qsg_batch_s* b) { return b->broken; }
 void qsgiDestroyBatch(qsg_batch_s* b) { destroyBatch(b); }
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums) { batchStep(b, balances, fluxSums); }
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t) { b->tune = t; }
@@ -465,10 +516,20 @@ int qsg_batch_step(qsg_batch b, qsg_balance* balances, double* flux_sums) {
     })
 }
 
+int qsg_batch_problem_error(qsg_batch b, int index, int* code) {
+    QSG_GUARD({
+        if (!b || !code || index < 0 || index >= (int)b->probs.size()) throw QsgError(QSG_ERR_ARG, "bad argument");
+        *code = b->slotError[index];
+    })
+}
+
 int qsg_batch_set_active(qsg_batch b, int index, int active) {
     QSG_GUARD({
         if (!b || index < 0 || index >= (int)b->probs.size()) throw QsgError(QSG_ERR_ARG, "bad batch index");
         if (active && !b->probs[index]) throw QsgError(QSG_ERR_STATE, "batch slot holds no problem");
+        if (active && b->slotError[index])
+            throw QsgError(QSG_ERR_STATE, "the slot's problem was taken out of the batch after an error");
+        if (active) qsgi::requireUsable(b->probs[index]);
         hipCheck(hipSetDevice(b->device), "hipSetDevice");
         uint32_t a = active ? 1u : 0u;
         hipCheck(hipStreamSynchronize(b->stream), "hipStreamSynchronize");

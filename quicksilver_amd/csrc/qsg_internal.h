@@ -32,6 +32,13 @@ struct PBuf {
 // Records an error message for qsg_last_error() and returns its code.
 int fail(const std::exception& e);
 
+// The device error flags (PassCounters::error, BatchState::error): bit 0
+// census overflow, 1 secondary overflow, 2 population-control overflow,
+// 3 runaway history, 4 more than 4 fission products, 5 census + source over
+// the processing buffer.  Their text and the ABI code they map to.
+std::string deviceErrorText(uint32_t flags);
+int deviceErrorCode(uint32_t flags);
+
 // Balance::Add (Tallies.hh:84-99): every counter summed.
 inline void addBalance(qsg_balance& acc, const qsg_balance& b) {
     acc.start += b.start;
@@ -92,6 +99,10 @@ struct qsg_problem_s {
     int censusBuf = -1;        // proc index holding the last census
     int phase = 0;             // 0 ready for init, 1 init done, 2 tracking done
     int cycle = 0;
+    // this cycle's processing population came from the caller
+    // (qsg_cycle_begin_host): no device source or population control ran
+    bool hostCycle = false;
+    uint64_t hostCount = 0;
 
     uint32_t* sortCounts = nullptr;     // [nCells * kSortBands]
     uint32_t* sortBlockSums = nullptr;  // scan scratch
@@ -121,6 +132,10 @@ struct qsg_problem_s {
     double lastTrackingSeconds = 0.0;
     qsg_stats stats = {};
     qsg_balance cumulative = {};  // Tallies::_balanceCumulative (Tallies.cc:65)
+    // set when a cycle stopped part way: the device state is no longer a
+    // cycle boundary and the problem takes no further cycle
+    int failCode = 0;
+    std::string failMsg;
 
     qs::LaunchCtx ctx() const { return qs::LaunchCtx{(void*)stream, numCUs}; }
 
@@ -147,14 +162,20 @@ namespace qsgi {
 // qsg_tuning <-> launch shape, with range checks (throws QSG_ERR_ARG).
 qs::TrackTuning tuningFromC(const qsg_tuning* t);
 void tuningToC(const qs::TrackTuning& in, qsg_tuning* out);
-// Particle-buffer capacity and total device bytes a problem will allocate.
-uint32_t particleCapacity(const qs::ProblemTables& T);
-uint64_t deviceBytesEstimate(const qs::ProblemTables& T);
+// Particle-buffer capacity (entries; `requested` 0 selects the default,
+// 2 x nParticles + one cycle's source + 65536) and the total device bytes a
+// problem will allocate with it.
+uint32_t particleCapacity(const qs::ProblemTables& T, uint64_t requested = 0);
+uint64_t deviceBytesEstimate(const qs::ProblemTables& T, uint64_t requested = 0);
 // Uploads already-built tables and allocates the device state.
 // ownStacks: allocate wave stacks for the problem's own tracking launches
 // (problems that only run inside a batch use the batch's stacks).
 qsg_problem_s* createFromTables(const qs::Parameters& params, qs::ProblemTables tables, int device,
-                                bool ownStacks);
+                                bool ownStacks, uint64_t capacity = 0);
+// Marks p failed (first failure wins) with the error's code and message.
+void markFailed(qsg_problem_s* p, int code, const std::string& msg);
+// Throws QSG_ERR_STATE if p failed earlier.
+void requireUsable(const qsg_problem_s* p);
 void destroy(qsg_problem_s* p);
 }  // namespace qsgi
 
@@ -170,6 +191,10 @@ void qsgiDestroyBatch(qsg_batch_s* b);
 void qsgiBatchStep(qsg_batch_s* b, qsg_balance* balances, double* fluxSums);
 void qsgiBatchSetTuning(qsg_batch_s* b, const qs::TrackTuning& t);
 const qsg_stats& qsgiBatchStats(const qsg_batch_s* b);
+// The error a slot's problem was taken out of the batch with (0: none).
+int qsgiBatchSlotError(const qsg_batch_s* b, int slot);
+// Whether a HIP failure ended the batch (no further step runs).
+bool qsgiBatchBroken(const qsg_batch_s* b);
 
 #define QSG_GUARD(...)                   \
     try {                                \

@@ -26,12 +26,14 @@ struct Request {
     qs::Parameters params;
     qs::ProblemTables tables;
     uint64_t bytes = 0;
+    uint64_t capacity = 0;  // particle-buffer entries asked for (0: the default)
     int nSteps = 0;
-    int state = 0;  // 0 queued, 1 active, 2 done, 3 failed
+    int state = 0;  // 0 queued, 1 active, 2 done, 3 failed, 4 cancelled
     qsg_problem_s* prob = nullptr;
     std::vector<qsg_balance> bal;
     std::vector<double> flux;
     std::string error;
+    int errorCode = 0;  // the QSG_ERR_* a failed request reports
     int slot = -1;  // batch slot while running
 };
 
@@ -60,7 +62,11 @@ void dropBatch(qsg_server_s* s) {
     s->batch = nullptr;
 }
 
-void retire(qsg_server_s* s, Request* r, int state) {
+void retire(qsg_server_s* s, Request* r, int state, int code = 0, const std::string& error = std::string()) {
+    if (state == 3) {
+        r->errorCode = code ? code : QSG_ERR_DEVICE;
+        r->error = error;
+    }
     if (r->slot >= 0 && s->batch) {
         try {
             qsgiBatchSetMember(s->batch, r->slot, nullptr);
@@ -107,6 +113,7 @@ void admit(qsg_server_s* s) {
         if (r->bytes > s->budget) {
             s->queue.pop_front();
             r->error = "request needs more device memory than the whole budget";
+            r->errorCode = QSG_ERR_CAPACITY;
             r->state = 3;
             s->stats.rejected++;
             continue;
@@ -115,7 +122,7 @@ void admit(qsg_server_s* s) {
         if (s->maxActive > 0 && (int)s->active.size() >= s->maxActive) break;
         s->queue.pop_front();
         try {
-            r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device, false);
+            r->prob = qsgi::createFromTables(r->params, std::move(r->tables), s->device, false, r->capacity);
             r->tables = qs::ProblemTables();
             s->used += r->bytes;
             const int slot = freeSlot(s);
@@ -130,34 +137,72 @@ void admit(qsg_server_s* s) {
                 r->prob = nullptr;
                 s->used -= r->bytes;
             }
+            const qsgi::QsgError* q = dynamic_cast<const qsgi::QsgError*>(&e);
             r->error = e.what();
+            r->errorCode = q ? q->code : QSG_ERR_DEVICE;
             r->state = 3;
             s->stats.rejected++;
         }
     }
 }
 
+// The running batch holds every active request.  It can be missing while
+// requests run: retiring one may drop it after a device error, and a
+// rebuild for more slots may fail after the old batch is gone.  The active
+// problems are then still at a cycle boundary, so a new batch takes them.
+void ensureBatch(qsg_server_s* s) {
+    if (s->batch || s->active.empty()) return;
+    std::vector<qsg_problem> hs;
+    for (Request* r : s->active) hs.push_back(r->prob);
+    int cap = s->maxActive > 0 ? s->maxActive : 16;
+    while (cap < (int)hs.size()) cap *= 2;
+    s->batch = qsgiCreateBatch(hs.data(), (int)hs.size(), cap, s->device, &s->stacks);
+    if (s->haveTune) qsgiBatchSetTuning(s->batch, s->tune);
+    for (size_t i = 0; i < s->active.size(); i++) s->active[i]->slot = (int)i;
+}
+
+void failAllActive(qsg_server_s* s, int code, const std::string& msg) {
+    for (Request* r : s->active) retire(s, r, 3, code, msg);
+    s->active.clear();
+    dropBatch(s);
+}
+
 void serverStep(qsg_server_s* s) {
     admit(s);
     if (s->active.empty()) return;
+    try {
+        ensureBatch(s);
+    } catch (const std::exception& e) {
+        const qsgi::QsgError* q = dynamic_cast<const qsgi::QsgError*>(&e);
+        failAllActive(s, q ? q->code : QSG_ERR_DEVICE, std::string("no batch for the running requests: ") + e.what());
+        throw;
+    }
     const int cap = qsgiBatchCapacity(s->batch);
     std::vector<qsg_balance> bal(cap);
     std::vector<double> flux(cap);
     try {
         qsgiBatchStep(s->batch, bal.data(), flux.data());
-    } catch (...) {
-        // a failed step leaves the problems' device state undefined
-        for (Request* r : s->active) {
-            r->error = "batch step failed";
-            retire(s, r, 3);
+    } catch (const qsgi::QsgError& e) {
+        if (qsgiBatchBroken(s->batch) || (e.code != QSG_ERR_CAPACITY && e.code != QSG_ERR_NUMERIC)) {
+            // a failed step leaves every problem's device state undefined
+            failAllActive(s, e.code, std::string("batch step failed: ") + e.what());
+            throw;
         }
-        s->active.clear();
-        dropBatch(s);
+        // some problems were taken out of the batch; the others advanced
+    } catch (const std::exception& e) {
+        failAllActive(s, QSG_ERR_DEVICE, std::string("batch step failed: ") + e.what());
         throw;
     }
     s->stats.steps++;
     std::vector<Request*> still;
     for (Request* r : s->active) {
+        if (const int code = qsgiBatchSlotError(s->batch, r->slot)) {
+            int c = 0;
+            char msg[512];
+            qsg_problem_error(r->prob, &c, msg, sizeof msg);
+            retire(s, r, 3, code, msg);
+            continue;
+        }
         r->bal.push_back(bal[r->slot]);
         r->flux.push_back(flux[r->slot]);
         s->stats.segments += bal[r->slot].num_segments;
@@ -169,6 +214,26 @@ void serverStep(qsg_server_s* s) {
         }
     }
     s->active.swap(still);
+}
+
+// Stops a request between steps: a queued one leaves the queue, a running
+// one frees its slot and device memory (it stands at a cycle boundary).
+void cancel(qsg_server_s* s, Request* r) {
+    if (r->state == 0) {
+        for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+            if (*it == r) {
+                s->queue.erase(it);
+                break;
+            }
+        r->state = 4;
+    } else if (r->state == 1) {
+        for (auto it = s->active.begin(); it != s->active.end(); ++it)
+            if (*it == r) {
+                s->active.erase(it);
+                break;
+            }
+        retire(s, r, 4);
+    }
 }
 
 Request* find(qsg_server_s* s, uint64_t id) {
@@ -197,7 +262,7 @@ int qsg_server_create(int device, uint64_t hbm_budget_bytes, int max_active, qsg
 int qsg_server_destroy(qsg_server s) {
     QSG_GUARD({
         if (!s) return QSG_OK;
-        for (Request* r : s->active) retire(s, r, 3);
+        for (Request* r : s->active) retire(s, r, 4);
         dropBatch(s);
         (void)hipSetDevice(s->device);
         s->stacks.release();
@@ -206,6 +271,10 @@ int qsg_server_destroy(qsg_server s) {
 }
 
 int qsg_server_submit(qsg_server s, const char* deck_text, uint64_t* request_id) {
+    return qsg_server_submit_opts(s, deck_text, nullptr, request_id);
+}
+
+int qsg_server_submit_opts(qsg_server s, const char* deck_text, const qsg_options* opts, uint64_t* request_id) {
     QSG_GUARD({
         if (!s || !deck_text || !request_id) throw QsgError(QSG_ERR_ARG, "NULL argument");
         std::unique_ptr<Request> r(new Request());
@@ -217,7 +286,8 @@ int qsg_server_submit(qsg_server s, const char* deck_text, uint64_t* request_id)
         qs::parseDeckText(deck_text, r->params);
         qs::supplyDefaults(r->params);
         r->tables = qs::buildProblem(r->params);
-        r->bytes = qsgi::deviceBytesEstimate(r->tables);
+        r->capacity = opts ? opts->particle_capacity : 0;
+        r->bytes = qsgi::deviceBytesEstimate(r->tables, r->capacity);
         r->nSteps = r->tables.nSteps;
         r->id = s->nextId++;
         *request_id = r->id;
@@ -250,8 +320,8 @@ int qsg_server_result(qsg_server s, uint64_t id, qsg_balance* balances, double* 
     QSG_GUARD({
         if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
         Request* r = find(s, id);
-        if (r->state == 3) throw QsgError(QSG_ERR_STATE, "request failed: " + r->error);
-        if (n > (int)r->bal.size()) throw QsgError(QSG_ERR_ARG, "more cycles requested than completed");
+        if (r->state == 3) throw QsgError(r->errorCode ? r->errorCode : QSG_ERR_STATE, "request failed: " + r->error);
+        if (n < 0 || n > (int)r->bal.size()) throw QsgError(QSG_ERR_ARG, "more cycles requested than completed");
         for (int i = 0; i < n; i++) {
             if (balances) balances[i] = r->bal[i];
             if (flux_sums) flux_sums[i] = r->flux[i];
@@ -259,18 +329,20 @@ int qsg_server_result(qsg_server s, uint64_t id, qsg_balance* balances, double* 
     })
 }
 
+int qsg_server_cancel(qsg_server s, uint64_t id) {
+    QSG_GUARD({
+        if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
+        Request* r = find(s, id);
+        if (r->state != 0 && r->state != 1) throw QsgError(QSG_ERR_STATE, "request is not queued or running");
+        cancel(s, r);
+    })
+}
+
 int qsg_server_release(qsg_server s, uint64_t id) {
     QSG_GUARD({
         if (!s) throw QsgError(QSG_ERR_ARG, "NULL server");
         Request* r = find(s, id);
-        if (r->state == 1) throw QsgError(QSG_ERR_STATE, "request is still running");
-        if (r->state == 0) {
-            for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
-                if (*it == r) {
-                    s->queue.erase(it);
-                    break;
-                }
-        }
+        cancel(s, r);
         s->all.erase(id);
     })
 }

@@ -1,0 +1,121 @@
+"""The qs driver's command-line and output surface against the reference's.
+
+Fixtures (tests/golden/outputs/, written by the reference run with
+``-e spectrum -S xs``, see make_golden.py): the parameters it prints
+(printParameters, Parameters.cc:97-215), the cross-section table it writes
+at set-up (initMC.cc:72 -> checkCrossSections, initMC.cc:392-484) and the
+census energy spectrum it writes at the end (gameOver, main.cc:87-94 ->
+EnergySpectrum.cc:37-58).
+
+* CPU: qs prints the same parameters byte for byte, and its printed deck is
+  a valid input that prints itself again ("the output is a valid input",
+  README.md:62-64); the cross-section table is written before any device
+  work, so it is checked here too.
+* GPU: a full run prints one line per cycle whose tallies equal the
+  reference's (bit-exact integers, flux within 1e-10), a figure-of-merit line
+  that is its own segments over its own tracking time (MC_Fast_Timer.cc:
+  97-104), and a spectrum file identical to the reference's; running the
+  printed deck gives the same cycles again.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, load_golden
+import quicksilver_amd as q
+
+QS = q.QS_EXE
+OUT = os.path.join(GOLDEN, "outputs")
+DECKS = ["small_cube_short", "nested_small", "nonflat"]
+HEADER = re.compile(r"^cycle\s+start\s+source")
+
+
+def _qs(args, cwd, timeout=300):
+    if not os.path.exists(QS):
+        pytest.fail(f"{QS} is not built; run __graft_entry__.build()")
+    return subprocess.run([QS, *args], capture_output=True, text=True, cwd=str(cwd), timeout=timeout)
+
+
+def _split(stdout):
+    """(printed parameters, per-cycle rows, figure-of-merit line)."""
+    lines = stdout.splitlines(keepends=True)
+    head = next((k for k, l in enumerate(lines) if HEADER.match(l)), len(lines))
+    rows, fom = [], None
+    for l in lines[head + 1:]:
+        tok = l.split()
+        if len(tok) == 15 and tok[0].isdigit():
+            rows.append(tok)
+        elif l.startswith("Figure Of Merit"):
+            fom = l
+    return "".join(lines[:head]), rows, fom
+
+
+def _deck_here(name, cwd):
+    """The fixture deck copied into the working directory as <name>.inp, the
+    way the reference ran it (the printed parameters name the input file)."""
+    import shutil
+    shutil.copy(load_golden(name)["deck_path"], os.path.join(str(cwd), name + ".inp"))
+    return name + ".inp"
+
+
+def _ref_text(name, suffix):
+    with open(os.path.join(OUT, f"{name}_{suffix}")) as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_printed_parameters_and_xs_table_match_reference(name, tmp_path):
+    r = _qs(["-i", _deck_here(name, tmp_path), "-e", "spectrum", "-S", "xs"], tmp_path)
+    printed, _, _ = _split(r.stdout)
+    assert printed == _ref_text(name, "params.txt")
+    with open(tmp_path / "xs.dat") as f:
+        assert f.read() == _ref_text(name, "xs.dat")
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_printed_deck_is_a_valid_input(name, tmp_path):
+    first, _, _ = _split(_qs(["-i", _deck_here(name, tmp_path)], tmp_path).stdout)
+    (tmp_path / "printed.inp").write_text(first)
+    again, _, _ = _split(_qs(["-i", "printed.inp"], tmp_path).stdout)
+    # the same deck again; only the inputFile line names the file it came from
+    assert f"   inputFile: {name}.inp\n" in first
+    assert again == first.replace(f"   inputFile: {name}.inp\n", "   inputFile: printed.inp\n")
+
+
+def _check_rows(name, rows, cycles):
+    assert len(rows) == len(cycles), name
+    for c, (tok, ref) in enumerate(zip(rows, cycles)):
+        assert int(tok[0]) == c
+        for f, v in zip(q.BALANCE_FIELDS[:12], tok[1:13]):
+            assert int(v) == ref[f], f"{name} cycle {c} {f}: qs {v} reference {ref[f]}"
+        assert abs(float(tok[13]) - ref["flux_sum"]) <= 1e-10 * abs(ref["flux_sum"]), (name, c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", DECKS)
+def test_qs_run_output_matches_reference(name, tmp_path):
+    print(f"[qs] deck={name} -e spectrum -S xs, then the printed deck", flush=True)
+    g = load_golden(name)
+    r = _qs(["-i", _deck_here(name, tmp_path), "-e", "spectrum", "-S", "xs"], tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    printed, rows, fom = _split(r.stdout)
+    _check_rows(name, rows, g["cycles"])
+    # FOM = segments / summed tracking time of the rows printed
+    m = re.search(r"Figure Of Merit\s+(\S+) .*\((\d+) segments, (\S+) s\)", fom)
+    assert m, fom
+    seg, secs = int(m.group(2)), float(m.group(3))
+    assert seg == sum(int(t[12]) for t in rows)
+    assert abs(secs - sum(float(t[14]) for t in rows)) <= 1e-5 * secs + 1e-12
+    assert abs(float(m.group(1)) - seg / secs) <= 0.01 * seg / secs
+    with open(tmp_path / "spectrum.dat") as f:
+        assert f.read() == _ref_text(name, "spectrum.dat")
+    with open(tmp_path / "xs.dat") as f:
+        assert f.read() == _ref_text(name, "xs.dat")
+    # the printed deck, run again, gives the same cycles
+    (tmp_path / "printed.inp").write_text(printed)
+    r2 = _qs(["-i", "printed.inp"], tmp_path)
+    assert r2.returncode == 0, r2.stderr[-2000:]
+    _, rows2, _ = _split(r2.stdout)
+    assert [t[:14] for t in rows2] == [t[:14] for t in rows]

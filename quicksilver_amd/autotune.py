@@ -15,8 +15,8 @@ Method: coordinate descent from the current best.  Each knob in turn is swept
 over its candidate values with the others fixed, scoring every candidate by
 segments per second of device time (HIP events on the library's stream)
 over a few cycles of the workload itself: tracking plus cycle set-up for
-one problem (so a sort pays for itself), the captured first half of each
-step for a batch.  Each candidate runs on a fresh copy of the workload
+one problem (so a sort pays for itself), the whole step (set-up, tracking
+passes and flux reduction) for a batch.  Each candidate runs on a fresh copy of the workload
 over the same cycles, so every candidate does identical work (the tallies
 do not depend on the shape).  The sweep repeats until a full round changes
 nothing (at most ``rounds`` rounds).  Results are cached in a JSON table
@@ -26,6 +26,8 @@ Usage:
     python -m quicksilver_amd.autotune deck.inp                 (plain form)
     python -m quicksilver_amd.autotune --batch a.inp b.inp ...  (batched form)
     python -m quicksilver_amd.autotune --mixed-batch 64         (the 64-problem mix)
+    python -m quicksilver_amd.autotune --mixed-batch 64 --sequence
+                                        (the same mix one problem at a time)
 """
 from __future__ import annotations
 
@@ -66,6 +68,8 @@ def signature(kind: str, decks: Sequence[str]) -> str:
         n, p = sizes[0]
         return f"plain:n{n}:p{p}"
     tot = sum(p for _, p in sizes)
+    if kind == "sequence":
+        return f"sequence:{len(sizes)}:p{tot}"
     return f"batch:{len(sizes)}:p{tot}"
 
 
@@ -139,8 +143,9 @@ def tune_plain(deck_text: str, warmup: int = 2, cycles: int = 2, rounds: int = 2
 def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 2, rounds: int = 1,
                log: Callable[[str], None] = print) -> Tuple[Shape, float]:
     """As tune_plain, on a fresh batch of the decks per candidate.  Score:
-    segments per second of the timed steps' captured first halves (set-up,
-    source, population control, sort and the tracking launch)."""
+    segments per second of the timed steps' device time, first event to
+    last (set-up, source, population control, sort, tracking passes and
+    flux reduction)."""
 
     def score(s: Shape) -> float:
         probs = [q.Problem(t) for t in deck_texts]
@@ -156,10 +161,35 @@ def tune_batch(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 2, roun
         finally:
             for p in probs:
                 p.close()
-        dt = st["track_kernel_seconds"] - st0["track_kernel_seconds"]
+        dt = st["tracking_seconds"] - st0["tracking_seconds"]
         return (st["segments"] - st0["segments"]) / dt if dt > 0 else 0.0
 
     start = lookup(signature("batch", deck_texts)) or dict(DEFAULT)
+    return coordinate_descent(score, start, rounds, log)
+
+
+def tune_sequence(deck_texts: Sequence[str], warmup: int = 1, cycles: int = 2, rounds: int = 1,
+                  log: Callable[[str], None] = print) -> Tuple[Shape, float]:
+    """The plain form run one problem at a time over a whole set of decks
+    (the baseline the batched form is compared with): one shape for every
+    problem of the set.  Score: the set's segments per second of device time
+    in cycle set-up plus tracking, each problem on a fresh copy over the same
+    cycles."""
+
+    def score(s: Shape) -> float:
+        seg, t = 0, 0.0
+        for text in deck_texts:
+            with q.Problem(text) as p:
+                p.set_tuning(**s)
+                p.run(warmup)
+                p.reset_stats()
+                p.run(cycles)
+                st = p.stats()
+            seg += st["segments"]
+            t += st["track_kernel_seconds"] + st["init_seconds"]
+        return seg / t if t > 0 else 0.0
+
+    start = lookup(signature("sequence", deck_texts)) or dict(DEFAULT)
     return coordinate_descent(score, start, rounds, log)
 
 
@@ -169,18 +199,24 @@ def main():
     ap.add_argument("--batch", action="store_true", help="tune the batched form over all decks")
     ap.add_argument("--mixed-batch", type=int, default=0, metavar="N",
                     help="tune the batched form on the first N mixed_batch decks (quicksilver_amd.decks)")
+    ap.add_argument("--sequence", action="store_true",
+                    help="tune the plain form run one problem at a time over all decks (with --mixed-batch: "
+                         "the one-at-a-time baseline of that mix)")
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--no-store", action="store_true")
     a = ap.parse_args()
     if a.mixed_batch:
         from quicksilver_amd import decks as D
         texts = [d.text() for d in D.mixed_batch(count=a.mixed_batch)]
-        a.batch = True
+        a.batch = not a.sequence
     elif a.decks:
         texts = [open(d).read() for d in a.decks]
     else:
         ap.error("give deck files or --mixed-batch N")
-    if a.batch:
+    if a.sequence:
+        shape, sc = tune_sequence(texts, rounds=a.rounds)
+        sig = signature("sequence", texts)
+    elif a.batch:
         shape, sc = tune_batch(texts, rounds=a.rounds)
         sig = signature("batch", texts)
     else:

@@ -30,6 +30,12 @@
 //                          text: seed x y z vx vy vz E w tc nmfp cell
 //   QSREF_PRINT_PARAMS=1   print the parameters first, as main.cc:44 does,
 //                          then a line QSREF_PARAMS_END
+//   QSREF_PRINT_ENERGIES=1 print the energy group bounds: QSREF_ENERGY g E (%.17e)
+//   QSREF_INJECT=path      the first cycle tracks the particles in this file
+//                          instead of the sourced, population-controlled
+//                          ones (its balance then has start = n and source,
+//                          rr, split = 0): uint64 n, n x {x y z vx vy vz E w
+//                          tc nmfp} f64, n x seed u64, n x global cell i32
 // At the end the census energy spectrum is written when energySpectrum (-e)
 // is set, as gameOver does (main.cc:93); crossSectionsOut (-S) is written by
 // initMC itself (initMC.cc:72).
@@ -57,10 +63,12 @@
 #include "ParticleVaultContainer.hh"
 #include "ParticleVault.hh"
 #include "MC_Particle_Buffer.hh"
+#include "MC_Base_Particle.hh"
 #include "MC_Processor_Info.hh"
 #include "MC_Time_Info.hh"
 #include "MC_SourceNow.hh"
 #include "MC_Domain.hh"
+#include "NuclearData.hh"
 #include "CycleTracking.hh"
 #include "SendQueue.hh"
 #include "MC_Fast_Timer.hh"
@@ -204,6 +212,54 @@ void dump_census(MonteCarlo* mc, const char* path)
     fclose(f);
 }
 
+// Replaces the processing population of the cycle just initialised by the
+// particles of `path` (MC_Base_Particle fields, ParticleVaultContainer::
+// addProcessingParticle, ParticleVaultContainer.cc:318-334).
+void inject(MonteCarlo* mc, const char* path)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) { fprintf(stderr, "cannot read %s\n", path); exit(3); }
+    uint64_t n = 0;
+    if (fread(&n, sizeof n, 1, f) != 1) exit(3);
+    std::vector<double> v(10 * n);
+    std::vector<uint64_t> seed(n);
+    std::vector<int32_t> cell(n);
+    if (fread(v.data(), sizeof(double), 10 * n, f) != 10 * n || fread(seed.data(), 8, n, f) != n ||
+        fread(cell.data(), 4, n, f) != n)
+        exit(3);
+    fclose(f);
+    MC_Domain& dom = mc->domain[0];
+    std::vector<int> local(dom.cell_state.size(), -1);
+    for (int c = 0; c < (int)dom.cell_state.size(); c++) local[dom.cell_state[c]._id >> 32] = c;
+    ParticleVaultContainer& pvc = *mc->_particleVaultContainer;
+    for (uint64_t q = 0; q < pvc.processingSize(); q++) pvc.getTaskProcessingVault(q)->clear();
+    uint64_t fill = 0;
+    for (uint64_t i = 0; i < n; i++)
+    {
+        MC_Base_Particle b;
+        const double* p = &v[10 * i];
+        b.coordinate.x = p[0];
+        b.coordinate.y = p[1];
+        b.coordinate.z = p[2];
+        b.velocity.x = p[3];
+        b.velocity.y = p[4];
+        b.velocity.z = p[5];
+        b.kinetic_energy = p[6];
+        b.weight = p[7];
+        b.time_to_census = p[8];
+        b.num_mean_free_paths = p[9];
+        b.random_number_seed = seed[i];
+        b.identifier = seed[i];
+        b.species = 0;
+        b.domain = 0;
+        b.cell = local[cell[i]];
+        pvc.addProcessingParticle(b, fill);
+    }
+    Balance& bal = mc->_tallies->_balanceTask[0];
+    bal._start = n;
+    bal._source = bal._rr = bal._split = 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -215,6 +271,9 @@ int main(int argc, char** argv)
         std::cout << "QSREF_PARAMS_END" << std::endl;
     }
     mcco = initMC(params);
+    if (getenv("QSREF_PRINT_ENERGIES"))
+        for (size_t g = 0; g < mcco->_nuclearData->_energies.size(); g++)
+            printf("QSREF_ENERGY %d %.17e\n", (int)g, mcco->_nuclearData->_energies[g]);
 #ifdef QSREF_GPU_TRACKING
     if (qsg_shim_attach(mcco, argc, (const char* const*)argv, 0) != 0) return 2;
 #endif
@@ -233,6 +292,7 @@ int main(int argc, char** argv)
     for (int step = 0; step < nSteps; step++)
     {
         cycle_init(mcco, params.simulationParams.loadBalance != 0);
+        if (step == 0 && getenv("QSREF_INJECT")) inject(mcco, getenv("QSREF_INJECT"));
         auto t0 = std::chrono::steady_clock::now();
 #ifdef QSREF_GPU_TRACKING
         cycleTracking(mcco);

@@ -6,6 +6,7 @@ Fixtures written next to this script:
                             and scalar-flux sums (%.17e, as JSON floats)
   tiny_cube_flux.npy        per-(cell, group) scalar flux of the last cycle
   tiny_cube_census.npz      census particles of the last cycle
+  nested_small_flux.npz, nested_small_census.npz  the same for nested_small
   coral2_p*_fluence.npy     cumulative per-cell fluence of the CORAL2 decks
                             (their JSON also holds the reference's CORAL2
                             PASS/FAIL verdict lines)
@@ -17,6 +18,10 @@ reference sources exist).  Usage:
     python tests/golden/make_golden.py mixed_batch serve_stream
                                   (per-deck tallies of the generated
                                   64-problem mix and request-stream prefix)
+    python tests/golden/make_golden.py injected
+                                  (nested_tiny_injected.npz: crafted particles
+                                  tracked one cycle; tallies, flux bins and
+                                  census)
     python tests/golden/make_golden.py outputs
                                   (outputs/<deck>_{spectrum.dat,xs.dat,
                                   params.txt}: the files and printed deck of
@@ -42,6 +47,10 @@ QS_REF = os.path.join(ROOT, "oracle", "_ref", "qs_ref")
 FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce",
           "collision", "escape", "census", "num_segments", "end"]
 
+
+# Decks whose last cycle's per-(cell, group) flux and census particles are
+# kept too (QSREF_FLUX_DUMP / QSREF_CENSUS_DUMP)
+DUMP_DECKS = ("tiny_cube", "nested_small")
 
 # Input decks the reference ships (Examples/), copied verbatim as fixture
 # inputs: the CORAL2 benchmark problems at their one-rank size.
@@ -160,12 +169,97 @@ def run_outputs():
         print(name, "outputs", flush=True)
 
 
+# Crafted particles tracked for one cycle (QSREF_INJECT): the branches full
+# runs do not reach on their own.
+INJECT_DECK = "nested_tiny"
+INJECT_FAMILIES = ("forced", "nudged", "group_edge", "census_time", "ordinary")
+
+
+def craft_particles(energies, n_per=400, seed=17, n=6):
+    """(state [N,10] x y z vx vy vz E w tc nmfp, seed [N] u64, cell [N] i32,
+    family [N]) on the n^3 mesh of 1 cm cells."""
+    rng = np.random.default_rng(seed)
+    rest, light = 9.395656981095e+2, 2.99792458e+10
+    states, seeds, cells, fams = [], [], [], []
+    for fam in INJECT_FAMILIES:
+        m = n_per
+        ijk = rng.integers(0, n, (m, 3))
+        cell = ijk[:, 0] + n * (ijk[:, 1] + n * ijk[:, 2])
+        pos = ijk + rng.uniform(0.02, 0.98, (m, 3))
+        d = rng.normal(size=(m, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        E = np.exp(rng.uniform(np.log(energies[0]), np.log(energies[-1]), m))
+        tc = np.full(m, 1e-8)
+        nmfp = np.zeros(m)
+        if fam == "forced":
+            # MC_Segment_Outcome.cc:42-59: a negative number of mean free
+            # paths forces a collision at distance tinyDouble
+            nmfp = rng.choice([-1.0, -1000.0, -1e-300], m)
+        elif fam == "nudged":
+            # just outside the cell across one face, heading further out: no
+            # facet bounds the particle, so MCT.cc:487-536 nudges it inward
+            ax = rng.integers(0, 3, m)
+            side = rng.integers(0, 2, m)
+            rows = np.arange(m)
+            out = 10.0 ** rng.uniform(-9, -4, m)
+            pos[rows, ax] = ijk[rows, ax] + np.where(side == 1, 1.0 + out, -out)
+            d[rows, ax] = np.where(side == 1, 1.0, -1.0) * (0.5 + np.abs(d[rows, ax]))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+        elif fam == "group_edge":
+            # exactly on a group bound (NuclearData.cc:208-228 puts it in the
+            # upper group), on eMax, and below eMin
+            g = rng.integers(0, len(energies), m)
+            E = np.asarray(energies)[g]
+            low = rng.random(m) < 0.1
+            E[low] = energies[0] * rng.uniform(0.01, 0.99, low.sum())
+        elif fam == "census_time":
+            # MC_Load_Particle.cc:16-19: time to census <= 0 gains one time step
+            tc = rng.choice([0.0, -0.0, -1e-9, -5e-9, 1e-12], m)
+        speed = light * np.sqrt(1.0 - (rest * rest) / ((E + rest) * (E + rest)))
+        v = d * speed[:, None]
+        states.append(np.column_stack([pos, v, E, np.ones(m), tc, nmfp]))
+        seeds.append(rng.integers(1, 2 ** 63, m, dtype=np.int64).astype(np.uint64))
+        cells.append(cell.astype(np.int32))
+        fams += [fam] * m
+    return (np.concatenate(states), np.concatenate(seeds), np.concatenate(cells), np.array(fams))
+
+
+def run_injected():
+    deck = D.nested_boxes(n=6, n_particles=2000, n_steps=1)
+    path = os.path.join(HERE, "decks", INJECT_DECK + ".inp")
+    deck.write(path)
+    tmp = tempfile.mkdtemp()
+    env = dict(os.environ, OMP_NUM_THREADS="8", QSREF_PRINT_ENERGIES="1")
+    out = subprocess.run([QS_REF, "-i", path], capture_output=True, text=True, env=env, check=True,
+                         cwd=tmp).stdout
+    energies = [float(l.split()[2]) for l in out.splitlines() if l.startswith("QSREF_ENERGY ")]
+    state, seed, cell, fam = craft_particles(energies)
+    inj = os.path.join(tmp, "inject.bin")
+    with open(inj, "wb") as f:
+        f.write(np.uint64(len(seed)).tobytes())
+        f.write(np.ascontiguousarray(state, np.float64).tobytes())
+        f.write(seed.tobytes())
+        f.write(cell.tobytes())
+    env = {"QSREF_INJECT": inj, "QSREF_FLUX_DUMP": os.path.join(tmp, "flux.bin"),
+           "QSREF_CENSUS_DUMP": os.path.join(tmp, "census.txt")}
+    cycles, fom, _ = run_reference(path, env_extra=env)
+    flux = np.fromfile(env["QSREF_FLUX_DUMP"], dtype=np.float64).reshape(deck.n_cells, D.N_GROUPS)
+    cols = np.loadtxt(env["QSREF_CENSUS_DUMP"], dtype=str, ndmin=2)
+    np.savez_compressed(os.path.join(HERE, INJECT_DECK + "_injected.npz"),
+                        state=state, seed=seed, cell=cell, family=fam, energies=np.array(energies),
+                        balance=np.array([cycles[0][k] for k in FIELDS], dtype=np.uint64),
+                        flux_sum=cycles[0]["flux_sum"], flux=flux,
+                        census_seed=cols[:, 0].astype(np.uint64), census_state=cols[:, 1:11].astype(np.float64),
+                        census_cell=cols[:, 11].astype(np.int32))
+    print(INJECT_DECK, "injected", len(seed), "particles:", cycles[0], flush=True)
+
+
 def main(names):
     os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
     all_decks = fixture_decks()
     for name in list(names):
-        if name in ("mixed_batch", "serve_stream", "outputs"):
-            run_outputs() if name == "outputs" else run_workload_set(name)
+        if name in ("mixed_batch", "serve_stream", "outputs", "injected"):
+            {"outputs": run_outputs, "injected": run_injected}.get(name, lambda: run_workload_set(name))()
             names.remove(name)
             if not names:
                 return
@@ -180,7 +274,7 @@ def main(names):
             all_decks[name].write(path)
         env = {}
         tmp = tempfile.mkdtemp()
-        if name == "tiny_cube":
+        if name in DUMP_DECKS:
             env = {"QSREF_FLUX_DUMP": os.path.join(tmp, "flux.bin"),
                    "QSREF_CENSUS_DUMP": os.path.join(tmp, "census.txt")}
         if name in EXAMPLE_DECKS:
@@ -195,12 +289,15 @@ def main(names):
                     np.fromfile(env["QSREF_FLUENCE_DUMP"], dtype=np.float64))
         with open(os.path.join(HERE, name + ".json"), "w") as f:
             json.dump(record, f, indent=1)
-        if name == "tiny_cube":
+        if name in DUMP_DECKS:
             n_cells = all_decks[name].n_cells
             flux = np.fromfile(env["QSREF_FLUX_DUMP"], dtype=np.float64).reshape(-1, n_cells, D.N_GROUPS)
-            np.save(os.path.join(HERE, "tiny_cube_flux.npy"), flux[-1])
+            if name == "tiny_cube":
+                np.save(os.path.join(HERE, "tiny_cube_flux.npy"), flux[-1])
+            else:
+                np.savez_compressed(os.path.join(HERE, name + "_flux.npz"), flux=flux[-1])
             cols = np.loadtxt(env["QSREF_CENSUS_DUMP"], dtype=str, ndmin=2)
-            np.savez_compressed(os.path.join(HERE, "tiny_cube_census.npz"),
+            np.savez_compressed(os.path.join(HERE, name + "_census.npz"),
                                 seed=cols[:, 0].astype(np.uint64),
                                 state=cols[:, 1:11].astype(np.float64),
                                 cell=cols[:, 11].astype(np.int32))

@@ -13,8 +13,10 @@ offsets straddling the probe's margin, grazing and axis-parallel
 directions, and starting points on or just outside the cell's faces.
 
 The oracle's search is pinned, beyond the full-run fixtures, by the
-closed form of a ray leaving a box (distance and face), and by the
-invariant that each facet index stands for one triangle of its face.
+reference's own MCT_Nearest_Facet on the same rays (oracle/_ref/qs_ref_mct,
+the unmodified reference objects, bit for bit), by the closed form of a ray
+leaving a box (distance and face), and by the invariant that each facet
+index stands for one triangle of its face.
 """
 import ctypes
 import os
@@ -25,6 +27,7 @@ import pytest
 from conftest import ROOT
 
 LIB = os.path.join(ROOT, "oracle", "_build", "libqsoracle.so")
+QS_REF_MCT = os.path.join(ROOT, "oracle", "_ref", "qs_ref_mct")
 
 _DECK = """Simulation:
    dt: 1e-08
@@ -120,6 +123,26 @@ def oracle_rays(L, deck_path, xyz, d, cell):
                                   cell.ctypes.data_as(I), dist.ctypes.data_as(D), facet.ctypes.data_as(I),
                                   moved.ctypes.data_as(D))
     assert rc == 0, rc
+    return dist, facet, moved
+
+
+def reference_rays(deck_path, xyz, d, cell, tmp_path):
+    """The reference's MCT_Nearest_Facet on the rays (oracle/_ref/qs_ref_mct)."""
+    import subprocess
+    n = cell.size
+    rays = tmp_path / "rays.bin"
+    out = tmp_path / "out.bin"
+    with open(rays, "wb") as f:
+        f.write(np.uint64(n).tobytes())
+        f.write(np.ascontiguousarray(np.concatenate([xyz, d], axis=1), np.float64).tobytes())
+        f.write(np.ascontiguousarray(cell, np.int32).tobytes())
+    r = subprocess.run([QS_REF_MCT, "-i", deck_path, str(rays), str(out)], capture_output=True, text=True,
+                       cwd=str(tmp_path), timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    raw = open(out, "rb").read()
+    dist = np.frombuffer(raw[:8 * n], np.float64)
+    facet = np.frombuffer(raw[8 * n:12 * n], np.int32)
+    moved = np.frombuffer(raw[12 * n:], np.float64).reshape(n, 3)
     return dist, facet, moved
 
 
@@ -267,6 +290,22 @@ def _write_deck(tmp_path, name):
 # ------------------------------------------------------------- oracle pins ----
 
 @pytest.mark.parametrize("name", sorted(MESHES))
+def test_oracle_matches_reference_mct(oracle, tmp_path, name):
+    """Every ray family, bit for bit: the restatement's search against the
+    reference's own MCT_Nearest_Facet (distance, facet, nudged position)."""
+    if not os.path.exists(QS_REF_MCT):
+        pytest.skip("oracle/_ref/qs_ref_mct is built only where the reference sources are present")
+    xyz, d, cell, fam = make_rays(name, n_per_family=2000, seed=5)
+    deck = _write_deck(tmp_path, name)
+    dist_o, facet_o, moved_o = oracle_rays(oracle, deck, xyz, d, cell)
+    dist_r, facet_r, moved_r = reference_rays(deck, xyz, d, cell, tmp_path)
+    bad = (dist_o.view(np.uint64) != dist_r.view(np.uint64)) | (facet_o != facet_r) | \
+        (moved_o.view(np.uint64) != moved_r.view(np.uint64)).any(axis=1)
+    assert not bad.any(), {f: int((fam[bad] == f).sum()) for f in np.unique(fam[bad])}
+    assert (fam == "outside").sum() > 0 and (moved_r != xyz).any(), "the nudge path is exercised"
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
 def test_oracle_matches_ray_box_exit(oracle, tmp_path, name):
     """For a point inside the cell, the nearest facet is where the ray leaves
     the box: distance min over axes of (bound - x) / d, on that face.  Facet
@@ -332,7 +371,13 @@ def test_device_nearest_facet_matches_oracle(oracle, tmp_path, name):
     import quicksilver_amd as q
     print(f"[facet-probe] mesh={name}", flush=True)
     xyz, d, cell, fam = make_rays(name)
-    dist_o, facet_o, moved_o = oracle_rays(oracle, _write_deck(tmp_path, name), xyz, d, cell)
+    deck = _write_deck(tmp_path, name)
+    dist_o, facet_o, moved_o = oracle_rays(oracle, deck, xyz, d, cell)
+    if os.path.exists(QS_REF_MCT):
+        # the reference's own search, where its build travelled with the tree
+        dist_r, facet_r, moved_r = reference_rays(deck, xyz, d, cell, tmp_path)
+        assert np.array_equal(dist_r.view(np.uint64), dist_o.view(np.uint64))
+        assert np.array_equal(facet_r, facet_o) and np.array_equal(moved_r.view(np.uint64), moved_o.view(np.uint64))
     with q.Problem(deck_text(name)) as p:
         dist_g, facet_g, moved_g = p.nearest_facet(xyz, d, cell)
     bad = (dist_g.view(np.uint64) != dist_o.view(np.uint64)) | (facet_g != facet_o) | \

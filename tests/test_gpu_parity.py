@@ -76,14 +76,20 @@ def test_launch_shape_does_not_change_tallies(name, shape):
 
 
 @pytest.mark.gpu
-def test_tiny_cube_flux_per_bin_and_census_particles():
+@pytest.mark.parametrize("name", ["tiny_cube", "nested_small"])
+def test_flux_per_bin_and_census_particles(name):
     """Per-(cell, group) flux of the last cycle and every census particle
-    (matched by its 64-bit seed) against the reference's dumps."""
+    (matched by its 64-bit seed) against the reference's dumps: one material
+    on 5^3 cells, and three nested materials on 12^3 cells."""
     import os
     from conftest import GOLDEN
-    g = load_golden("tiny_cube")
-    ref_flux = np.load(os.path.join(GOLDEN, "tiny_cube_flux.npy"))
-    cen = np.load(os.path.join(GOLDEN, "tiny_cube_census.npz"))
+    print(f"[parity] deck={name}: flux bins and census of the last cycle", flush=True)
+    g = load_golden(name)
+    if name == "tiny_cube":
+        ref_flux = np.load(os.path.join(GOLDEN, "tiny_cube_flux.npy"))
+    else:
+        ref_flux = np.load(os.path.join(GOLDEN, name + "_flux.npz"))["flux"]
+    cen = np.load(os.path.join(GOLDEN, name + "_census.npz"))
     with q.Problem.from_file(g["deck_path"]) as p:
         n = p.info().n_steps
         p.run(n - 1)
@@ -93,6 +99,7 @@ def test_tiny_cube_flux_per_bin_and_census_particles():
         census = p.census()
         bal, _ = p.cycle_finalize()
     assert flux.shape == ref_flux.shape
+    assert (ref_flux != 0).sum() > 2000  # the sourced energies fill the top groups
     scale = np.abs(ref_flux).max()
     np.testing.assert_allclose(flux, ref_flux, rtol=1e-10, atol=1e-12 * scale)
     assert bal["census"] == len(cen["seed"]) == len(census)

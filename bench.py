@@ -176,10 +176,11 @@ def reference_threads(cpus: dict) -> int:
     return max(1, n)
 
 
-def run_reference(deck_text: str, threads: int):
-    """The reference CPU path (oracle/_ref/qs_ref, OpenMP) on one deck:
-    per-cycle balances, total segments and tracking seconds (its FOM
-    numerator and denominator, MC_Fast_Timer.cc:97-104)."""
+def run_reference(deck_text: str, threads: int, n_steps: int = 0):
+    """The reference CPU path (oracle/_ref/qs_ref, OpenMP) on one deck
+    (n_steps > 0: that many steps, the reference's -N): per-cycle balances,
+    total segments and tracking seconds (its FOM numerator and denominator,
+    MC_Fast_Timer.cc:97-104)."""
     fields = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce", "collision",
               "escape", "census", "num_segments", "end"]
     with tempfile.TemporaryDirectory() as tmp:
@@ -187,8 +188,9 @@ def run_reference(deck_text: str, threads: int):
         with open(deck, "w") as f:
             f.write(deck_text)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-        out = subprocess.run([QS_REF, "-i", deck], capture_output=True, text=True, env=env, cwd=tmp,
-                             timeout=900, check=True).stdout
+        cmd = [QS_REF, "-i", deck] + (["-N", str(n_steps)] if n_steps > 0 else [])
+        out = subprocess.run(cmd, capture_output=True, text=True, env=env, cwd=tmp, timeout=900,
+                             check=True).stdout
     bals, seg, secs = [], 0, 0.0
     for line in out.splitlines():
         tok = line.split()
@@ -201,20 +203,21 @@ def run_reference(deck_text: str, threads: int):
 
 def cpu_mixed_sample(ds, gpu_balances, warmup: int, threads: int, budget_segments: float):
     """The reference on the decks of the mix in order, each taken if its
-    segments (from the committed reference fixture of the mix) still fit the
-    budget; every step of each deck, one deck after another.  segments / sum
-    of tracking times.  Where the GPU's timed cycles fall inside the deck's
-    steps, the reference's tallies of those cycles are compared with them."""
-    with open(os.path.join(ROOT, "tests", "golden", "mixed_batch.json")) as f:
-        fx = json.load(f)["problems"]
+    segments (estimated from the GPU's timed cycles of the same deck) still
+    fit the budget; each run for the GPU's warm-up + timed steps (-N), one
+    deck after another.  segments / sum of tracking times.  The reference's
+    tallies of the timed cycles are compared with the GPU's."""
     seg, secs, used, same, compared = 0, 0.0, [], True, 0
     for i, d in enumerate(ds):
-        if seg + sum(c["num_segments"] for c in fx[i]["cycles"]) > budget_segments:
+        timed_gpu = gpu_balances[i]
+        steps = len(timed_gpu)
+        est = sum(b["num_segments"] for b in timed_gpu) * (warmup + steps) / max(steps, 1)
+        if seg + est > budget_segments:
             continue
-        bals, s, t = run_reference(d.text(), threads)
-        timed = bals[warmup:warmup + len(gpu_balances[i])]
-        if len(timed) == len(gpu_balances[i]):
-            same = same and timed == gpu_balances[i]
+        bals, s, t = run_reference(d.text(), threads, warmup + steps)
+        timed = bals[warmup:warmup + steps]
+        if len(timed) == steps:
+            same = same and timed == timed_gpu
             compared += 1
         seg += s
         secs += t
@@ -224,7 +227,8 @@ def cpu_mixed_sample(ds, gpu_balances, warmup: int, threads: int, budget_segment
     return {"value": seg / secs if secs > 0 else None, "unit": "segments/s", "cores": threads,
             "kind": "reference",
             "sample": f"{len(used)} of the {len(ds)} mixed_batch decks (indices {used}: taken in order while "
-                      f"their segments fit {budget_segments:.2g}), every step of each, run one after another "
+                      f"their segments fit {budget_segments:.2g}), each for the GPU's {warmup} warm-up + timed "
+                      "steps, one after another "
                       f"with {threads} OpenMP threads; {seg} segments / {secs:.3f} s of the reference's "
                       "tracking time (its FOM; excludes source and population control)",
             "segments": seg, "decks_compared_with_gpu": compared, "tallies_identical_to_gpu": same}

@@ -289,7 +289,8 @@ int qsg_nearest_facet_probe(qsg_problem p, uint64_t n, const double* xyz, const 
                             double* distance, int32_t* facet, double* xyz_out);
 
 /* Device FP64 builtins on host arrays (rounding checks).
- * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a) */
+ * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a)
+ *     7 the sine and 8 the cosine of one sincos(a) call */
 int qsg_math_probe(int op, const double* a, const double* b, double* out, uint64_t n, int device);
 
 #ifdef __cplusplus

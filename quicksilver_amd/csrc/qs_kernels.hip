@@ -2007,6 +2007,16 @@ __global__ void k_math(int op, const double* a, const double* b, double* out, si
         case 4: r = cos(x); break;
         case 5: r = x * y; break;
         case 6: r = (double)(*(const uint64_t*)&a[q]); break;
+        case 7: {
+            double c;
+            sincos(x, &r, &c);
+            break;
+        }
+        case 8: {
+            double sn;
+            sincos(x, &sn, &r);
+            break;
+        }
         default: r = 0;
     }
     out[q] = r;

@@ -130,6 +130,19 @@ def test_device_fp64_builtins_round_like_ieee():
         assert np.max(np.abs(d - h) / ulp) <= 2.0
 
 
+@pytest.mark.gpu
+def test_device_sincos_equals_sin_and_cos():
+    """The device's sincos gives bit for bit the device's sin and cos, on the
+    azimuth angles updateTrajectory draws (2 * 3.14159265 * rngSample,
+    CollisionEvent.cc:33-34) and on a wider range, so the rotation may take
+    both from one call."""
+    rng = np.random.default_rng(11)
+    r = rng.random(1 << 20)
+    for x in (2 * 3.14159265 * r, (r - 0.5) * 1e4):
+        np.testing.assert_array_equal(q.math_probe(7, x), q.math_probe(3, x))
+        np.testing.assert_array_equal(q.math_probe(8, x), q.math_probe(4, x))
+
+
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name,shape", [("nested_small", "default"), ("coral2_p2", "default"),

@@ -155,8 +155,12 @@ def test_qs_run_output_matches_reference(name, tmp_path):
     r2 = _qs(["-i", "printed.inp"], tmp_path)
     assert r2.returncode == 0, r2.stderr[-2000:]
     _, rows2, _ = _split(r2.stdout)
-    # the same tallies; the flux sum again within 1e-10 (the device's atomic
-    # adds make its last digits depend on the order of the adds)
+    # The printed deck is the reference's printParameters output, which writes
+    # every value at the stream's default 6 significant digits
+    # (Parameters.cc:97-215): nested_small's drawn cross sections (e.g.
+    # totalCrossSection 2.29916917591182 -> 2.29917) come back rounded, so
+    # the re-run is the same problem only to that precision.  Its balance
+    # tallies are the same on these decks, and its flux agrees to 1e-5.
     assert [t[:13] for t in rows2] == [t[:13] for t in rows]
     for a, b in zip(rows2, rows):
-        assert abs(float(a[13]) - float(b[13])) <= 1e-10 * abs(float(b[13]))
+        assert abs(float(a[13]) - float(b[13])) <= 1e-5 * abs(float(b[13]))

@@ -288,6 +288,18 @@ int qsg_get_fluence(qsg_problem p, double* fluence, uint64_t n_cells);
 int qsg_nearest_facet_probe(qsg_problem p, uint64_t n, const double* xyz, const double* dir, const int32_t* cell,
                             double* distance, int32_t* facet, double* xyz_out);
 
+/* No counterpart in the reference.  qsg_nearest_facet_probe for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same kernel and results, with
+ * the rays and results staged through one fixed device array per GPU
+ * (5.5 MiB, held until the process ends) instead of a per-call allocation.
+ * At most QSG_PROBE_NOALLOC_MAX_RAYS rays per call; more is QSG_ERR_ARG.
+ * Calls on one GPU share its array and are serialised by the library (each
+ * returns after its results are copied out). */
+#define QSG_PROBE_NOALLOC_MAX_RAYS 65536
+int qsg_nearest_facet_probe_noalloc(qsg_problem p, uint64_t n, const double* xyz, const double* dir,
+                                    const int32_t* cell, double* distance, int32_t* facet, double* xyz_out);
+
 /* Device FP64 builtins on host arrays (rounding checks).
  * op: 0 sqrt(a) 1 a/b 2 log(a) 3 sin(a) 4 cos(a) 5 a*b 6 (double)(uint64 bits of a)
  *     7 the sine and 8 the cosine of one sincos(a) call */

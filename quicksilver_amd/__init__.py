@@ -24,6 +24,9 @@ ERRORS = {1: "ARG", 2: "DEVICE", 3: "CAPACITY", 4: "STATE", 5: "NUMERIC"}
 BALANCE_FIELDS = ["start", "source", "rr", "split", "absorb", "scatter", "fission", "produce",
                   "collision", "escape", "census", "num_segments", "end"]
 
+#: rays per qsg_nearest_facet_probe_noalloc call (QSG_PROBE_NOALLOC_MAX_RAYS)
+NOALLOC_MAX_RAYS = 65536
+
 #: every entry point declared in include/qsg.h
 EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_create_from_deck_opts",
            "qsg_problem_error", "qsg_destroy", "qsg_get_info",
@@ -37,7 +40,8 @@ EXPORTS = ["qsg_last_error", "qsg_create", "qsg_create_from_deck", "qsg_create_f
            "qsg_server_submit", "qsg_server_submit_opts", "qsg_server_step", "qsg_server_status",
            "qsg_server_result", "qsg_server_cancel", "qsg_server_release", "qsg_server_set_tuning",
            "qsg_server_get_stats",
-           "qsg_coral_check", "qsg_get_fluence", "qsg_get_energy_spectrum", "qsg_nearest_facet_probe"]
+           "qsg_coral_check", "qsg_get_fluence", "qsg_get_energy_spectrum", "qsg_nearest_facet_probe",
+           "qsg_nearest_facet_probe_noalloc"]
 
 
 class QsgError(RuntimeError):
@@ -227,6 +231,7 @@ def lib() -> ctypes.CDLL:
     L.qsg_math_probe.argtypes = [i, _D, _D, _D, ctypes.c_uint64, i]
     _I32 = _P(ctypes.c_int32)
     L.qsg_nearest_facet_probe.argtypes = [vp, ctypes.c_uint64, _D, _D, _I32, _D, _I32, _D]
+    L.qsg_nearest_facet_probe_noalloc.argtypes = [vp, ctypes.c_uint64, _D, _D, _I32, _D, _I32, _D]
     L.qsg_get_energy_spectrum.argtypes = [vp, _D, _P(ctypes.c_uint64), ctypes.c_uint64]
     L.qsg_get_fluence.argtypes = [vp, _D, ctypes.c_uint64]
     L.qsg_get_stats.argtypes = [vp, _P(Stats)]
@@ -389,10 +394,11 @@ class Problem:
         _check(lib().qsg_get_fluence(self._h, out.ctypes.data_as(_D), out.size))
         return out
 
-    def nearest_facet(self, xyz: np.ndarray, direction: np.ndarray, cell: np.ndarray):
+    def nearest_facet(self, xyz: np.ndarray, direction: np.ndarray, cell: np.ndarray, noalloc: bool = False):
         """The tracking kernels' nearest-facet search on given rays (see
         qsg_nearest_facet_probe): returns (distance, facet, position after
-        any nudge)."""
+        any nudge).  noalloc: through qsg_nearest_facet_probe_noalloc (no
+        device allocation; at most NOALLOC_MAX_RAYS rays)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         direction = np.ascontiguousarray(direction, dtype=np.float64).reshape(-1, 3)
         cell = np.ascontiguousarray(cell, dtype=np.int32).reshape(-1)
@@ -403,9 +409,9 @@ class Problem:
         facet = np.empty(n, np.int32)
         moved = np.empty((n, 3), np.float64)
         i32 = ctypes.POINTER(ctypes.c_int32)
-        _check(lib().qsg_nearest_facet_probe(self._h, n, xyz.ctypes.data_as(_D), direction.ctypes.data_as(_D),
-                                             cell.ctypes.data_as(i32), dist.ctypes.data_as(_D),
-                                             facet.ctypes.data_as(i32), moved.ctypes.data_as(_D)))
+        fn = lib().qsg_nearest_facet_probe_noalloc if noalloc else lib().qsg_nearest_facet_probe
+        _check(fn(self._h, n, xyz.ctypes.data_as(_D), direction.ctypes.data_as(_D), cell.ctypes.data_as(i32),
+                  dist.ctypes.data_as(_D), facet.ctypes.data_as(i32), moved.ctypes.data_as(_D)))
         return dist, facet, moved
 
     def coral_check(self) -> CoralReport:

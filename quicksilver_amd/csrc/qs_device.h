@@ -57,6 +57,9 @@ struct MatInfo {
 
 constexpr int kMaxMaterials = 16;
 constexpr int kMaxReactions = 16;
+// Energy-group bounds (nGroups + 1) a batch tracking wave keeps in LDS; a
+// batch with a problem of more groups searches them in device memory.
+constexpr int kBatchStagedEnergies = 257;
 
 struct DevProblem {
     // grid
@@ -66,7 +69,6 @@ struct DevProblem {
     double hdx, hdy, hdz;  // dx/2.0 ...
     int boundaryMask;      // 2 bits per face +x -x +y -y +z -z: 1 escape, 2 reflect
     const double* planes;    // [nCells][24][A,B,C,D]
-    const double* planeND;   // [nCells][24][2]: the facet's one nonzero normal component and D
     const int32_t* cellMat;
     const double* cellVolume;
     const double* cellCenter;
@@ -74,6 +76,10 @@ struct DevProblem {
     // nuclear data
     int G;
     int nMat, maxR;
+    // The group bounds are log-spaced (NuclearData.cc:105-119): the group of
+    // energy E is about (log2 E - eLog2Lo) * eGroupsPerLog2, a first guess
+    // that the exact comparisons with `energies` then settle
+    float eLog2Lo, eGroupsPerLog2;
     const double* energies;  // [G+1]
     const double* total;     // [nMat][G]
     const double* mfp;       // [nMat][G] mean free path 1/total (1e75 where total is 0)
@@ -291,7 +297,8 @@ void launchSort(const LaunchCtx& ctx, const DevProblem& P, Particles in, const u
 void launchBatchSort(const LaunchCtx& ctx, BatchSlot* slots, int n, int interval, const uint64_t* keyBase,
                      uint64_t* sortPrefix, uint32_t* counts, size_t totalKeys, uint32_t* blockSums);
 void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
-                      const WaveStacks& ws);
+                      const WaveStacks& ws,
+                      bool stagedEnergies);
 void launchBatchFinalize(const LaunchCtx& ctx, BatchSlot* slots, int n, int partsPerProblem, double* fluxSums);
 
 // Device copies of single FP64 builtins, for rounding checks.

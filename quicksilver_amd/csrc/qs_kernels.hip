@@ -86,16 +86,25 @@ __device__ __forceinline__ uint64_t rng_spawn(uint64_t& parent) {
 // NuclearData::getEnergyGroup (NuclearData.cc:208-228).  The reference
 // returns G for E > eMax, one past the last table row; tables here have G
 // rows, so that case is clamped to G-1.
-__device__ __forceinline__ int energy_group(const double* e, int G, double E) {
+__device__ __forceinline__ int energy_group(const double* e, int G, double E, float log2Lo, float perLog2) {
     if (E <= e[0]) return 0;
     if (E > e[G]) return G - 1;
-    int high = G, low = 0;
-    while (high != low + 1) {
-        int mid = (high + low) / 2;
-        if (E < e[mid]) high = mid;
-        else low = mid;
+    if (E != E) return G - 1;  // what the bisection returns for NaN
+    // The bisection's answer is the largest g <= G-1 with e[g] <= E.  The
+    // bounds are log-spaced, so a single-precision log2 puts the guess within
+    // one group of it; the table's own bounds then decide exactly (the two
+    // loops run once at most, except for a table that is not log-spaced).
+    const float t = (__log2f((float)E) - log2Lo) * perLog2;
+    int g = (int)fminf(fmaxf(t, 0.0f), (float)(G - 1));
+    const double lo = e[g], hi = e[g + 1];
+    if (lo > E) {
+        g--;
+        while (g > 0 && e[g] > E) g--;
+    } else if (hi <= E && g < G - 1) {
+        g++;
+        while (g < G - 1 && e[g + 1] <= E) g++;
     }
-    return low;
+    return g;
 }
 
 // --------------------------------------------------------- geometry ----
@@ -168,6 +177,38 @@ __device__ __forceinline__ Node3 facet_node(const Cell9& c, uint32_t code, int m
     n.y = sel3((b >> 2) & 3u, c.Y0, c.Y1, c.Yh);
     n.z = sel3((b >> 4) & 3u, c.Z0, c.Z1, c.Zh);
     return n;
+}
+
+// The plane of a facet on a face of axis ax (0 x, 1 y, 2 z) from its nodes
+// r0, r1, r2, computed as MC_General_Plane does (MC_Facet_Geometry.hh:18-39)
+// operation by operation, reduced to what tracking reads: the nonzero normal
+// component N and D.  The facets are axis-aligned (the host checks every
+// plane when it builds the tables), so the two other components are signed
+// zeros: A*A + B*B + C*C is the nonzero component squared, exactly, and
+// A*x0 + B*y0 + C*z0 is the nonzero term whenever that term is nonzero.
+// When the term is zero only the sign of D's zero can differ, and a zero
+// plane sum is recomputed from the stored four-term plane
+// (facet_plane_distance, plane_sum).  No memory is read: the nodes come
+// from the cell's nine coordinates.
+__device__ __forceinline__ void plane_nd(int ax, const Node3& r0, const Node3& r1, const Node3& r2, double& N,
+                                         double& D) {
+    // (p, q) = the axes after ax in cyclic order: the nonzero component is
+    // (r1.p - r0.p)(r2.q - r0.q) - (r1.q - r0.q)(r2.p - r0.p)
+    const double p0 = ax == 0 ? r0.y : (ax == 1 ? r0.z : r0.x), q0 = ax == 0 ? r0.z : (ax == 1 ? r0.x : r0.y);
+    const double p1 = ax == 0 ? r1.y : (ax == 1 ? r1.z : r1.x), q1 = ax == 0 ? r1.z : (ax == 1 ? r1.x : r1.y);
+    const double p2 = ax == 0 ? r2.y : (ax == 1 ? r2.z : r2.x), q2 = ax == 0 ? r2.z : (ax == 1 ? r2.x : r2.y);
+    const double x0 = ax == 0 ? r0.x : (ax == 1 ? r0.y : r0.z);
+    const double c = ((p1 - p0) * (q2 - q0)) - ((q1 - q0) * (p2 - p0));
+    const double d = -1.0 * (c * x0);
+    const double inv = 1.0 / sqrt(c * c);
+    N = c * inv;
+    D = d * inv;
+}
+
+// N and D of facet f of the cell (see plane_nd).
+__device__ __forceinline__ void facet_nd(const Cell9& c9, int f, double& N, double& D) {
+    const uint32_t code = c_facetCode[f];
+    plane_nd(f >> 3, facet_node(c9, code, 0), facet_node(c9, code, 1), facet_node(c9, code, 2), N, D);
 }
 
 // Cell-local node triples of the 24 facets (MC_Domain.cc:42-47) resolved to
@@ -383,9 +424,9 @@ __device__ __forceinline__ double distance_to_face_facet(double planeTol, double
 }
 
 template <int AX, int T>
-__device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pnd, const double* pl,
-                                            const Cell9& c9, double planeTol, double x, double y, double z,
-                                            double a, double b, double g, bool plus, bool minus) {
+__device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
+                                            double x, double y, double z, double a, double b, double g, bool plus,
+                                            bool minus) {
     constexpr int FP = 8 * AX + T, FM = FP + 4;
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
@@ -395,8 +436,9 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pnd, 
         fn.add(kHuge, FM);
         return;
     }
-    const double* p = pnd + 2 * f;
-    const double N = p[0], D = p[1];
+    const uint32_t code = plus ? c_facetCode[FP] : c_facetCode[FM];
+    double N, D;
+    plane_nd(AX, facet_node(c9, code, 0), facet_node(c9, code, 1), facet_node(c9, code, 2), N, D);
     const double dot = N * (AX == 0 ? a : (AX == 1 ? b : g));
     double d = kHuge;
     if (dot > 0.0) {
@@ -412,15 +454,14 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pnd, 
 }
 
 template <int AX>
-__device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pnd, const double* pl, const Cell9& c9,
-                                           double planeTol, double x, double y, double z, double a, double b,
-                                           double g) {
+__device__ __forceinline__ void visit_axis(FindNearest& fn, const double* pl, const Cell9& c9, double planeTol,
+                                           double x, double y, double z, double a, double b, double g) {
     const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
-    visit_facet<AX, 0>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 1>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 2>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
-    visit_facet<AX, 3>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 0>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 1>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 2>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
+    visit_facet<AX, 3>(fn, pl, c9, planeTol, x, y, z, a, b, g, plus, minus);
 }
 
 // One facet of any face, for the rare retry path: facets are visited in
@@ -543,11 +584,12 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const Cell9& c9, double x, 
 // to its containment test), for a facet chosen per lane at run time: kHuge
 // when the direction does not leave through the plane or the particle lies
 // behind it beyond the plane tolerance.
-__device__ __forceinline__ double facet_plane_distance(int f, const double* pnd, const double* pl, double planeTol,
+__device__ __forceinline__ double facet_plane_distance(int f, const Cell9& c9, const double* pl, double planeTol,
                                                        double x, double y, double z, double a, double b,
                                                        double g) {
     const int ax = f >> 3;
-    const double N = pnd[2 * f], D = pnd[2 * f + 1];
+    double N, D;
+    facet_nd(c9, f, N, D);
     const double dAx = ax == 0 ? a : (ax == 1 ? b : g);
     const double dot = N * dAx;
     if (!(dot > 0.0)) return kHuge;
@@ -567,12 +609,11 @@ __device__ __forceinline__ double facet_plane_distance(int f, const double* pnd,
 // per lane at run time: the projection axes are selected per lane, so the
 // whole wave runs one copy of the test whatever face each lane's candidate
 // lies on.
-__device__ __forceinline__ double exact_facet_test(int f, const double* pnd, const double* pl, const Cell9& c9,
-                                                   double planeTol, double x, double y, double z, double a,
-                                                   double b, double g) {
+__device__ __forceinline__ double exact_facet_test(int f, const double* pl, const Cell9& c9, double planeTol,
+                                                   double x, double y, double z, double a, double b, double g) {
     const int ax = f >> 3;
     const double tol = 1e-9;
-    const double distance = facet_plane_distance(f, pnd, pl, planeTol, x, y, z, a, b, g);
+    const double distance = facet_plane_distance(f, c9, pl, planeTol, x, y, z, a, b, g);
     if (distance == kHuge) return kHuge;
     double ix = x + distance * a;
     double iy = y + distance * b;
@@ -611,8 +652,7 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pnd, con
 // other facet is kHuge.  A lane whose probe cannot decide runs the exact
 // test on all 12 exit-face facets.  The rare retries use a compact loop
 // over all 24 facets so they add no register pressure to the common path.
-__device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, const double* pnd,
-                                                 int cell, const Cell9& c9, double& x, double& y, double& z,
+__device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell, const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
     Nearest nf;
     {
@@ -640,14 +680,14 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
             // the only candidate, its exit point deeper inside it than
             // margin: the reference's containment test passes for it, so
             // only the distance is computed
-            const double d = facet_plane_distance(q0, pnd, pl, planeTol, x, y, z, a, b, g);
+            const double d = facet_plane_distance(q0, c9, pl, planeTol, x, y, z, a, b, g);
             fn.add(d, q0);
             fn.add(kHuge, (d != kHuge && q0 == 23) ? 22 : 23);
         } else if (unsure || n > 3) {
             // the reference's test on the 12 facets of the three exit faces
-            visit_axis<0>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
-            visit_axis<1>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
-            visit_axis<2>(fn, pnd, pl, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<1>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+            visit_axis<2>(fn, pl, c9, planeTol, x, y, z, a, b, g);
         } else {
             // exact tests of the candidates, merged in facet order; every
             // other facet is kHuge, which only sets the largest kHuge index
@@ -655,7 +695,7 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
 #pragma unroll 1
             for (int i = 0; i < n; i++) {
                 const int f = i == 0 ? q0 : (i == 1 ? q1 : q2);
-                const double d = exact_facet_test(f, pnd, pl, c9, planeTol, x, y, z, a, b, g);
+                const double d = exact_facet_test(f, pl, c9, planeTol, x, y, z, a, b, g);
                 fn.add(d, f);
                 const bool fin = d != kHuge;
                 if (i == 0) fin0 = fin;
@@ -839,6 +879,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // Tables an event needs, staged in LDS when they fit.
 struct Tables {
+    float eLog2Lo, eGroupsPerLog2;  // the group guess of energy_group
     const double* energies;
     const double* total;
     const double* mfp;
@@ -850,6 +891,8 @@ struct Tables {
 template <bool kLds>
 __device__ __forceinline__ Tables stage_tables(const DevProblem& P, double* lds) {
     Tables t;
+    t.eLog2Lo = P.eLog2Lo;
+    t.eGroupsPerLog2 = P.eGroupsPerLog2;
     if (!kLds) {
         t.energies = P.energies;
         t.total = P.total;
@@ -920,6 +963,9 @@ __device__ __forceinline__ uint64_t rfl64(uint64_t v) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return ((uint64_t)hi << 32) | lo;
 }
+__device__ __forceinline__ float rflf(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
 __device__ __forceinline__ double rfld(double v) {
     return __builtin_bit_cast(double, rfl64(__builtin_bit_cast(uint64_t, v)));
 }
@@ -945,7 +991,6 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.hdz = rfld(c.hdz);
     P.boundaryMask = rfl(c.boundaryMask);
     P.planes = rflp(c.planes);
-    P.planeND = rflp(c.planeND);
     P.cellMat = rflp(c.cellMat);
     P.cellVolume = rflp(c.cellVolume);
     P.cellCenter = rflp(c.cellCenter);
@@ -953,6 +998,8 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     P.G = rfl(c.G);
     P.nMat = rfl(c.nMat);
     P.maxR = rfl(c.maxR);
+    P.eLog2Lo = rflf(c.eLog2Lo);
+    P.eGroupsPerLog2 = rflf(c.eGroupsPerLog2);
     P.energies = rflp(c.energies);
     P.total = rflp(c.total);
     P.mfp = rflp(c.mfp);
@@ -969,6 +1016,9 @@ __device__ __forceinline__ DevProblem load_problem(const DevProblem& c) {
     return P;
 }
 
+// kStageE: every problem's energy-group bounds fit the wave's LDS slot
+// (kBatchStagedEnergies), and are read from there.
+template <bool kStageE>
 struct BatchQueue {
     static constexpr bool kMulti = true;
     const BatchSlot* slots;
@@ -977,14 +1027,21 @@ struct BatchQueue {
     int p;
     MatInfo* sMat;     // this wave's LDS copy of the current problem's materials
     int32_t* sReact;   // ... and reaction types
+    double* sE;        // ... and energy-group bounds (when they fit)
     __device__ __forceinline__ DevProblem prob() const { return load_problem(slots[p].prob); }
-    // Copies the current problem's material records and reaction types into
-    // the wave's LDS slots (read on every collision).
+    // Copies the current problem's material records, reaction types (read on
+    // every collision) and energy-group bounds (read by every group search)
+    // into the wave's LDS slots.
     __device__ __forceinline__ void stage() const {
         const DevProblem& P = slots[p].prob;
         const int nM = rfl(P.nMat), nR = rfl(P.nMat * P.maxR);
         for (int q = __lane_id(); q < nM; q += 64) sMat[q] = P.mat[q];
         for (int q = __lane_id(); q < nR; q += 64) sReact[q] = P.reactType[q];
+        if (kStageE) {
+            const int nE = rfl(P.G) + 1;
+            const double* e = rflp(P.energies);
+            for (int q = __lane_id(); q < nE; q += 64) sE[q] = e[q];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -992,7 +1049,9 @@ struct BatchQueue {
     __device__ __forceinline__ Tables tables() const {
         const DevProblem& P = slots[p].prob;
         Tables T;
-        T.energies = rflp(P.energies);
+        T.eLog2Lo = rflf(P.eLog2Lo);
+        T.eGroupsPerLog2 = rflf(P.eGroupsPerLog2);
+        T.energies = kStageE ? (const double*)sE : rflp(P.energies);
         T.total = rflp(P.total);
         T.mfp = rflp(P.mfp);
         T.macro = rflp(P.macro);
@@ -1198,7 +1257,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 if (L.tc <= 0.0) L.tc += P.dt;
             }
         }
-        if (grp) L.group = energy_group(T.energies, G, L.E);
+        if (grp) L.group = energy_group(T.energies, G, L.E, T.eLog2Lo, T.eGroupsPerLog2);
         upd = rec = grp = ld = false;
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
@@ -1221,7 +1280,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         double dCen = L.speed * L.tc;
         const Cell9 c9 = cell_nodes(P, L.ci, L.cj, L.ck);
         const double* pl = P.planes + (size_t)cell * 96;
-        Nearest nf = nearest_facet(P, pl, P.planeND + (size_t)cell * 48, cell, c9, L.x, L.y, L.z, L.a, L.b, L.g);
+        Nearest nf = nearest_facet(P, pl, cell, c9, L.x, L.y, L.z, L.a, L.b, L.g);
         double dFacet = nf.d;
         if (force) {
             dFacet = kHuge;
@@ -2037,7 +2096,7 @@ __global__ void __launch_bounds__(256) k_nearest_probe(DevProblem P, const doubl
     double x = xyz[3 * q], y = xyz[3 * q + 1], z = xyz[3 * q + 2];
     const double a = dir[3 * q], b = dir[3 * q + 1], g = dir[3 * q + 2];
     const Cell9 c9 = cell_nodes(P, ci, cj, ck);
-    const Nearest nf = nearest_facet(P, P.planes + (size_t)c * 96, P.planeND + (size_t)c * 48, c, c9, x, y, z, a, b, g);
+    const Nearest nf = nearest_facet(P, P.planes + (size_t)c * 96, c, c9, x, y, z, a, b, g);
     dist[q] = nf.d;
     facet[q] = nf.f;
     xyzOut[3 * q] = x;
@@ -2064,7 +2123,7 @@ __global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux
 // (the same search MC_Load_Particle does) scaled to kSortBands.
 
 __device__ __forceinline__ uint32_t sort_key(const DevProblem& P, const Particles& in, uint32_t i) {
-    const int g = energy_group(P.energies, P.G, in.E()[i]);
+    const int g = energy_group(P.energies, P.G, in.E()[i], P.eLog2Lo, P.eGroupsPerLog2);
     return (uint32_t)in.cell()[i] * kSortBands + (uint32_t)(g * kSortBands / P.G);
 }
 
@@ -2391,14 +2450,15 @@ __global__ void __launch_bounds__(256) k_batch_sort_scatter(BatchSlot* slots, in
 // problem and moves round-robin through the problems, draining one
 // problem's queue at a time (so its descriptor and tables stay
 // wave-uniform); it exits after a full round finds no queued particle.
-template <int kMinWaves>
+template <int kMinWaves, bool kStageE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
 k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune, WaveStacks ws) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     __shared__ MatInfo sMat[4][kMaxMaterials];
     __shared__ int32_t sReact[4][kMaxMaterials * kMaxReactions];
-    BatchQueue acc = {slots, n, passKind, (int)(wave % (uint32_t)n), sMat[threadIdx.x >> 6],
-                      sReact[threadIdx.x >> 6]};
+    __shared__ double sE[4][kStageE ? kBatchStagedEnergies : 1];
+    BatchQueue<kStageE> acc = {slots, n, passKind, (int)(wave % (uint32_t)n), sMat[threadIdx.x >> 6],
+                               sReact[threadIdx.x >> 6], sE[threadIdx.x >> 6]};
     if (!slots[acc.p].st->active && !acc.next()) return;
     run_queue(acc, tune, ws);
 }
@@ -2612,12 +2672,12 @@ void launchBatchPassPrep(const LaunchCtx& ctx, BatchSlot* slots, int n, int pass
 }
 
 void launchBatchTrack(const LaunchCtx& ctx, BatchSlot* slots, int n, int passKind, const TrackTuning& tune,
-                      const WaveStacks& ws) {
-    decltype(&k_batch_track<3>) kern;
+                      const WaveStacks& ws, bool stagedEnergies) {
+    decltype(&k_batch_track<3, true>) kern;
     switch (tune.wavesPerSimd) {
-        case 2: kern = k_batch_track<2>; break;
-        case 4: kern = k_batch_track<4>; break;
-        default: kern = k_batch_track<3>; break;
+        case 2: kern = stagedEnergies ? k_batch_track<2, true> : k_batch_track<2, false>; break;
+        case 4: kern = stagedEnergies ? k_batch_track<4, true> : k_batch_track<4, false>; break;
+        default: kern = stagedEnergies ? k_batch_track<3, true> : k_batch_track<3, false>; break;
     }
     hipLaunchKernelGGL(kern, dim3((unsigned)batchTrackBlocks(ctx, tune)), dim3(tune.blockSize), 0,
                        (hipStream_t)ctx.stream, slots, n, passKind, tune, ws);

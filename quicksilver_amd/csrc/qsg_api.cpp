@@ -152,20 +152,6 @@ void build(qsg_problem_s* p, int device, uint64_t capacity) {
     d.boundaryMask = 0;
     for (int f = 0; f < 6; f++) d.boundaryMask |= (T.boundary[f] & 3) << (2 * f);
     d.planes = p->upload(T.planes);
-    {
-        // The planes are axis-aligned (checked when the tables are built):
-        // facet f of a cell has B = C = 0 (x faces, f < 8), A = C = 0 (y) or
-        // A = B = 0 (z).  The tracking loop reads only the nonzero component
-        // and D, in a compact copy.
-        std::vector<double> nd((size_t)T.nCells * 24 * 2);
-        for (size_t c = 0; c < (size_t)T.nCells; c++)
-            for (int f = 0; f < 24; f++) {
-                const double* pl = &T.planes[(c * 24 + f) * 4];
-                nd[(c * 24 + f) * 2 + 0] = pl[f / 8];
-                nd[(c * 24 + f) * 2 + 1] = pl[3];
-            }
-        d.planeND = p->upload(nd);
-    }
     d.cellMat = p->upload(T.cellMaterial);
     d.cellVolume = p->upload(T.cellVolume);
     d.cellCenter = p->upload(T.cellCenter);
@@ -174,6 +160,14 @@ void build(qsg_problem_s* p, int device, uint64_t capacity) {
     d.nMat = (int)T.materials.size();
     d.maxR = T.maxReactions;
     d.energies = p->upload(T.energies);
+    // energy_group's first guess: the bounds are exp(logLow + i * delta),
+    // delta = (logHigh - logLow) / (G + 1) (NuclearData.cc:105-119), so the
+    // group of E is about (log2 E - log2 e[0]) * ln 2 / delta
+    {
+        const double logLow = std::log(T.energies.front()), logHigh = std::log(T.energies.back());
+        d.eLog2Lo = (float)(logLow / std::log(2.0));
+        d.eGroupsPerLog2 = logHigh > logLow ? (float)(std::log(2.0) * (T.nGroups + 1.0) / (logHigh - logLow)) : 0.0f;
+    }
     std::vector<double> total((size_t)d.nMat * d.G), mfp((size_t)d.nMat * d.G),
         macro((size_t)d.nMat * d.maxR * d.G, 0.0);
     std::vector<int32_t> react((size_t)d.nMat * d.maxR, 0);
@@ -774,7 +768,7 @@ uint64_t qsgi::deviceBytesEstimate(const qs::ProblemTables& T, uint64_t requeste
     b += (G + 1) * 8 + 2 * M * G * 8 + M * R * G * 8 + M * R * 4;
     const uint64_t cap = particleCapacity(T, requested);
     b += 5 * (cap * qs::kParticleBytes + 12 * 256) + qs::popGroups((uint32_t)cap) * sizeof(qs::PopGroup);
-    b += nc * 24 * 16 + qs::sortKeys((int)nc) * 4 + qs::scanBlocks(qs::sortKeys((int)nc)) * 4;
+    b += qs::sortKeys((int)nc) * 4 + qs::scanBlocks(qs::sortKeys((int)nc)) * 4;
     return b + 64 * 1024;
 }
 

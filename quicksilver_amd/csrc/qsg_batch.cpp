@@ -28,6 +28,10 @@ struct qsg_batch_s {
     size_t sortKeys = 0;
     size_t sortKeysCap = 0;
     bool graphsValid = false;  // cleared when memory the graphs hold is reallocated
+    // every member's energy-group bounds fit the tracking waves' LDS slots
+    // (qs::kBatchStagedEnergies); the captured graphs hold the kernel chosen
+    // for it
+    bool stagedEnergies = true;
     double* dFluxSums = nullptr;
     int partsPerProblem = 64;  // blocks per problem for zeroing its flux bins
     qs::BatchState* hState = nullptr;  // pinned mirrors
@@ -106,8 +110,15 @@ void checkMember(const qsg_batch_s* b, const qsg_problem_s* p) {
 void uploadKeyBase(qsg_batch_s* b) {
     const int n = (int)b->probs.size();
     std::vector<uint64_t> keyBase(n + 1, 0);
-    for (int i = 0; i < n; i++)
+    bool staged = true;
+    for (int i = 0; i < n; i++) {
         keyBase[i + 1] = keyBase[i] + (b->probs[i] ? qs::sortKeys(b->probs[i]->tables.nCells) : 0);
+        if (b->probs[i] && b->probs[i]->tables.nGroups + 1 > qs::kBatchStagedEnergies) staged = false;
+    }
+    if (staged != b->stagedEnergies) {
+        b->stagedEnergies = staged;
+        b->graphsValid = false;
+    }
     b->sortKeys = keyBase[n];
     if (b->sortKeys > b->sortKeysCap || !b->dSortCounts) {
         for (void* d : {(void*)b->dSortCounts, (void*)b->dSortBlockSums})
@@ -275,7 +286,7 @@ void enqueueSetup(qsg_batch_s* b, const qs::LaunchCtx& c) {
 // The first tracking pass.  Captured as a graph of its own, so that events
 // recorded between graph launches time the tracking kernel alone.
 void enqueueFirstPass(qsg_batch_s* b, const qs::LaunchCtx& c, const qs::WaveStacks& ws) {
-    qs::launchBatchTrack(c, b->dSlots, (int)b->probs.size(), 0, b->tune, ws);
+    qs::launchBatchTrack(c, b->dSlots, (int)b->probs.size(), 0, b->tune, ws, b->stagedEnergies);
     hipCheck(hipGetLastError(), "batch track launch");
 }
 
@@ -404,7 +415,7 @@ void stepMembers(qsg_batch_s* b, qsg_balance* balances, double* fluxSums, int& f
         hipEvent_t e0 = passEvent(b, 2 * k), e1 = passEvent(b, 2 * k + 1);
         qs::launchBatchPassPrep(c, b->dSlots, n, kind);
         hipCheck(hipEventRecord(e0, b->stream), "hipEventRecord");
-        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune, ws);
+        qs::launchBatchTrack(c, b->dSlots, n, kind, b->tune, ws, b->stagedEnergies);
         hipCheck(hipGetLastError(), "batch track launch");
         hipCheck(hipEventRecord(e1, b->stream), "hipEventRecord");
         readStates(b);

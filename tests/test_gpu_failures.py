@@ -5,13 +5,16 @@ CPU path's tallies for its deck.  The reference itself only prints an
 assertion and continues (qs_assert.hh:9-26), so the reports are this build's
 own; the overflow is forced from the host (a small per-problem particle
 capacity), and every device store past a capacity is skipped by a bounds
-check in the kernels, so each kernel exits normally."""
+check in the kernels, so each kernel exits normally.  The numeric error
+(QSG_ERR_NUMERIC) comes from a deck whose fission multiplicity can exceed
+the reference's MAX_PRODUCTION_SIZE of 4 (CollisionEvent.cc:15, 100-104):
+the kernel keeps 4 products, flags the cycle and finishes it."""
 import pytest
 
 import quicksilver_amd as q
 from conftest import load_golden
 
-CAPACITY, STATE = 3, 4
+CAPACITY, STATE, NUMERIC = 3, 4, 5
 
 
 def _check(name, g, got, n=None):
@@ -157,3 +160,48 @@ def test_server_isolates_failed_request_and_cancels():
         with pytest.raises(q.QsgError):
             s.status(rid)
         assert s.stats()["bytes_in_use"] == 0
+
+
+def _too_many_products(g):
+    """nuBar 4.5: (int)(nuBar + r) is 5 for half the fissions
+    (NuclearData.cc:71), more than MAX_PRODUCTION_SIZE."""
+    text = open(g["deck_path"]).read()
+    assert text.count("nuBar: 2.0") == 1
+    return text.replace("nuBar: 2.0", "nuBar: 4.5")
+
+
+@pytest.mark.gpu
+def test_plain_run_too_many_fission_products_is_numeric_error():
+    print("[failures] plain qsg_run, fission multiplicity above 4", flush=True)
+    g = load_golden("small_cube_short")
+    with q.Problem(_too_many_products(g)) as p:
+        with pytest.raises(q.QsgError) as e:
+            p.run(1)
+        assert e.value.code == NUMERIC
+        code, msg = p.error()
+        assert code == NUMERIC and "more than 4" in msg
+        with pytest.raises(q.QsgError) as e2:
+            p.run(1)
+        assert e2.value.code == STATE
+
+
+@pytest.mark.gpu
+def test_batch_isolates_numeric_error():
+    print("[failures] batch with one problem over the production limit", flush=True)
+    good = load_golden("tiny_cube")
+    bad = load_golden("small_cube_short")
+    probs = [q.Problem.from_file(good["deck_path"]), q.Problem(_too_many_products(bad))]
+    try:
+        with q.Batch(probs) as b:
+            out = []
+            first = b.step(strict=False)
+            assert first[1] is None
+            assert b.problem_error(1) == NUMERIC and b.problem_error(0) == 0
+            out.append(first[0])
+            for _ in range(1, len(good["cycles"])):
+                out.append(b.step()[0])
+        _check("tiny_cube", good, out)
+        assert probs[1].error()[0] == NUMERIC
+    finally:
+        for p in probs:
+            p.close()

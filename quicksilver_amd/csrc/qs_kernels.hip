@@ -113,6 +113,14 @@ __device__ __forceinline__ int energy_group(const double* e, int G, double E, fl
 // tuples (GlobalFccGrid.cc:47-69) and the facet node triples
 // (MC_Domain.cc:42-47) by facetNodeCodes() on the host side of this file.
 __constant__ uint32_t c_facetCode[24];
+// The same codes in LDS, for the lookups indexed per lane (a facet chosen at
+// run time): an LDS read instead of a memory load on the tracking loop's
+// critical path.  Every kernel that tracks fills it (load_facet_codes, then a
+// barrier) before its first search.
+__shared__ uint32_t s_facetCode[24];
+__device__ __forceinline__ void load_facet_codes() {
+    if (threadIdx.x < 24) s_facetCode[threadIdx.x] = c_facetCode[threadIdx.x];
+}
 
 struct Cell9 {
     double X0, X1, Xh, Y0, Y1, Yh, Z0, Z1, Zh;
@@ -207,7 +215,7 @@ __device__ __forceinline__ void plane_nd(int ax, const Node3& r0, const Node3& r
 
 // N and D of facet f of the cell (see plane_nd).
 __device__ __forceinline__ void facet_nd(const Cell9& c9, int f, double& N, double& D) {
-    const uint32_t code = c_facetCode[f];
+    const uint32_t code = s_facetCode[f];
     plane_nd(f >> 3, facet_node(c9, code, 0), facet_node(c9, code, 1), facet_node(c9, code, 2), N, D);
 }
 
@@ -270,6 +278,16 @@ constexpr EdgeTable makeEdgeTable() {
 }
 
 constexpr EdgeTable kEdgeTri = makeEdgeTable();
+
+// kEdgeTri[AX] packed 2 bits per entry at bit 2 * (4 * side + edge), so a
+// lane's triangle comes from a shift of a constant, not a table load.
+template <int AX>
+constexpr uint32_t edgeTriPacked() {
+    uint32_t v = 0;
+    for (int side = 0; side < 2; side++)
+        for (int e = 0; e < 4; e++) v |= (uint32_t)kEdgeTri.tri[AX][side][e] << (2 * (4 * side + e));
+    return v;
+}
 
 // Value of one node coordinate for a compile-time code.
 template <int AXIS, int CODE>
@@ -566,13 +584,14 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const Cell9& c9, double x, 
         int other = -1;
         if (near1) other = s2 > 0.0 ? (wedge == 1 ? 3 : 1) : (wedge == 2 ? 0 : 2);
         else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
-        const int t1 = kEdgeTri.tri[AX][side][wedge];
+        constexpr uint32_t kTri = edgeTriPacked<AX>();
+        const int t1 = (int)((kTri >> (2 * (4 * side + wedge))) & 3u);
         r.c1 = base + t1;
         // no second candidate: farther than margin from both diagonals;
         // and farther than margin inside the face's own edges
         r.deep = other < 0 && fabs(du) <= (hu - margin) * dot0 && fabs(dv) <= (hv - margin) * dot0;
         if (other >= 0) {
-            const int t2 = kEdgeTri.tri[AX][side][other];
+            const int t2 = (int)((kTri >> (2 * (4 * side + other))) & 3u);
             r.c1 = base + (t1 < t2 ? t1 : t2);
             r.c2 = base + (t1 < t2 ? t2 : t1);
         }
@@ -620,7 +639,7 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pl, cons
     double iz = z + distance * g;
     const double iu = ax == 0 ? iy : (ax == 1 ? iz : ix);
     const double iv = ax == 0 ? iz : (ax == 1 ? ix : iy);
-    const uint32_t code = c_facetCode[f];
+    const uint32_t code = s_facetCode[f];
     const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
     const double p0u = ax == 0 ? n0.y : (ax == 1 ? n0.z : n0.x), p0v = ax == 0 ? n0.z : (ax == 1 ? n0.x : n0.y);
     const double p1u = ax == 0 ? n1.y : (ax == 1 ? n1.z : n1.x), p1v = ax == 0 ? n1.z : (ax == 1 ? n1.x : n1.y);
@@ -1552,6 +1571,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
     extern __shared__ double lds[];
     __shared__ MatInfo sMat[kMaxMaterials];
     __shared__ int32_t sReact[kMaxMaterials * kMaxReactions];
+    load_facet_codes();
     Tables T = stage_tables<kLds>(P, lds);
     for (int q = threadIdx.x; q < P.nMat; q += blockDim.x) sMat[q] = P.mat[q];
     for (int q = threadIdx.x; q < P.nMat * P.maxR; q += blockDim.x) sReact[q] = P.reactType[q];
@@ -2089,6 +2109,8 @@ __global__ void __launch_bounds__(256) k_nearest_probe(DevProblem P, const doubl
                                                        const int32_t* __restrict__ cell, double* __restrict__ dist,
                                                        int32_t* __restrict__ facet, double* __restrict__ xyzOut,
                                                        size_t n) {
+    load_facet_codes();
+    __syncthreads();
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int c = cell[q];
@@ -2453,6 +2475,8 @@ __global__ void __launch_bounds__(256) k_batch_sort_scatter(BatchSlot* slots, in
 template <int kMinWaves, bool kStageE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWaves)))
 k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune, WaveStacks ws) {
+    load_facet_codes();
+    __syncthreads();
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     __shared__ MatInfo sMat[4][kMaxMaterials];
     __shared__ int32_t sReact[4][kMaxMaterials * kMaxReactions];

@@ -1180,54 +1180,63 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         uint64_t idle = __ballot(!active);
         if ((top != 0 || !exhausted) && idle != 0 &&
             (__popcll(idle) >= (uint32_t)tune.refillMin || idle == ~0ull)) {
-            while (idle != 0) {
-                const bool fromStack = top != 0;
-                uint32_t avail;
-                if (fromStack) {
-                    avail = top;
-                } else {
-                    if (exhausted) break;
-                    if (qBase >= qEnd) {
-                        uint32_t got = 0;
-                        if (lane == 0) got = atomicAdd(io.head, (uint32_t)tune.chunk);
-                        got = __shfl(got, 0);
-                        if (got >= nIn) {
-                            exhausted = true;
-                            break;
-                        }
-                        qBase = got;
-                        qEnd = min(got + (uint32_t)tune.chunk, nIn);
+            // Every idle lane's source is decided first, then one load per
+            // field serves all of them: loads issued in separate rounds
+            // into the same registers would each wait for the previous
+            // round's to land.  The stack gives its top entries first; then
+            // the rest of the current queue chunk (range A), then one new
+            // claim of at least a chunk (range B).
+            const uint32_t nIdle = __popcll(idle);
+            const uint32_t takeS = min(top, nIdle);
+            uint32_t need = nIdle - takeS;
+            const uint32_t aBase = qBase;
+            uint32_t aN = 0, bBase = 0, bN = 0;
+            if (need != 0 && !exhausted) {
+                aN = min(qEnd - qBase, need);
+                need -= aN;
+                qBase += aN;
+                if (need != 0) {
+                    const uint32_t want = max(need, (uint32_t)tune.chunk);
+                    uint32_t got = 0;
+                    if (lane == 0) got = atomicAdd(io.head, want);
+                    got = __shfl(got, 0);
+                    if (got >= nIn) {
+                        exhausted = true;
+                    } else {
+                        const uint32_t end = min(got + want, nIn);
+                        bBase = got;
+                        bN = min(end - got, need);
+                        qBase = got + bN;
+                        qEnd = end;
                     }
-                    avail = qEnd - qBase;
                 }
-                const Particles src = fromStack ? stack : io.in;
-                const uint32_t nIdle = __popcll(idle);
-                const uint32_t take = min(avail, nIdle);
-                const uint32_t rank = __popcll(idle & lt);
-                if (!active && rank < take) {
-                    // MC_Load_Particle (MC_Load_Particle.cc:11-29) +
-                    // MC_Particle(const MC_Base_Particle&) (MC_Base_Particle.hh:281-320)
-                    const uint32_t idx = fromStack ? top - 1 - rank : qBase + rank;
-                    L.x = src.x()[idx];
-                    L.y = src.y()[idx];
-                    L.z = src.z()[idx];
-                    L.vx = src.vx()[idx];
-                    L.vy = src.vy()[idx];
-                    L.vz = src.vz()[idx];
-                    L.E = src.E()[idx];
-                    L.w = src.w()[idx];
-                    L.tc = src.tc()[idx];
-                    L.nmfp = src.nmfp()[idx];
-                    L.seed = src.seed()[idx];
-                    L.ci = src.cell()[idx];  // decoded after the refill
-                    ld = true;
-                    histSeg = 0;
-                    active = true;
-                }
-                if (fromStack) top -= take;
-                else qBase += take;
-                idle = __ballot(!active);
             }
+            const uint32_t rank = __popcll(idle & lt);
+            if (!active && rank < takeS + aN + bN) {
+                // MC_Load_Particle (MC_Load_Particle.cc:11-29) +
+                // MC_Particle(const MC_Base_Particle&) (MC_Base_Particle.hh:281-320)
+                const bool fromStack = rank < takeS;
+                const uint32_t r = rank - takeS;
+                const uint32_t idx = fromStack ? top - 1 - rank : (r < aN ? aBase + r : bBase + (r - aN));
+                const Particles src = {fromStack ? stack.base : io.in.base,
+                                       fromStack ? stack.stride : io.in.stride};
+                L.x = src.x()[idx];
+                L.y = src.y()[idx];
+                L.z = src.z()[idx];
+                L.vx = src.vx()[idx];
+                L.vy = src.vy()[idx];
+                L.vz = src.vz()[idx];
+                L.E = src.E()[idx];
+                L.w = src.w()[idx];
+                L.tc = src.tc()[idx];
+                L.nmfp = src.nmfp()[idx];
+                L.seed = src.seed()[idx];
+                L.ci = src.cell()[idx];  // decoded after the refill
+                ld = true;
+                histSeg = 0;
+                active = true;
+            }
+            top -= takeS;
             if (lane == 0) *sTopW = top;
         }
         // room for this iteration's pushes (uniform: decided before any lane diverges)

@@ -1359,25 +1359,24 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 const double m0 = mrow[0];
                 const double m1 = mi.nReact > 1 ? mrow[G] : 0.0;
                 const double m2 = mi.nReact > 2 ? mrow[2 * G] : 0.0;
-                for (int iso = 0; iso < mi.nIso; iso++) {
-                    cur -= m0;
-                    if (cur < 0) {
-                        selR = 0;
-                        break;
-                    }
-                    if (mi.nReact > 1) {
-                        cur -= m1;
-                        if (cur < 0) {
-                            selR = 1;
-                            break;
-                        }
-                    }
-                    if (mi.nReact > 2) {
-                        cur -= m2;
-                        if (cur < 0) {
-                            selR = 2;
-                            break;
-                        }
+                const bool has1 = mi.nReact > 1, has2 = mi.nReact > 2;
+                // One isotope per step as straight-line code: the same
+                // subtractions in the same order, the first partial sum
+                // below zero selecting the reaction; a lane that has
+                // selected, or scanned all its isotopes, keeps its values
+                // while the wave finishes the others (a uniform loop, with
+                // no per-lane branch to leave it).
+                for (int iso = 0;; iso++) {
+                    const bool live = selR < 0 && iso < mi.nIso;
+                    if (__ballot(live) == 0) break;
+                    const double c1 = cur - m0;
+                    const double c2 = has1 ? c1 - m1 : c1;
+                    const double c3 = has2 ? c2 - m2 : c2;
+                    if (live) {
+                        if (c1 < 0) selR = 0;
+                        else if (has1 && c2 < 0) selR = 1;
+                        else if (has2 && c3 < 0) selR = 2;
+                        else cur = c3;
                     }
                 }
             } else {

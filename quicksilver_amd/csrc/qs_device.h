@@ -249,9 +249,6 @@ void launchPopulation(const LaunchCtx& ctx, const DevProblem& P, Particles in, u
 // (pc->nCensus), secondaries to `extra` (*nExtraOut).
 // Blocks a tracking pass over nIn queued particles launches (persistent
 // grid of tune.wavesPerCU waves per CU, fewer for small queues).
-// One-time device set-up (constant tables) for launches on ctx's device;
-// call before the first launch and outside any stream capture.
-int prepareDevice(const LaunchCtx& ctx);  // returns a hipError_t
 bool tablesFitLds(const DevProblem& P);
 long long trackPassBlocks(const LaunchCtx& ctx, const TrackTuning& tune, uint32_t nIn);
 long long batchTrackBlocks(const LaunchCtx& ctx, const TrackTuning& tune);

@@ -24,8 +24,6 @@
 // is set (the host then finishes the held cycle itself).
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-
 #include <cmath>
 #include <cstdint>
 
@@ -108,19 +106,11 @@ __device__ __forceinline__ int energy_group(const double* e, int G, double E, fl
 }
 
 // --------------------------------------------------------- geometry ----
-// Per-facet node codes: for each of the facet's 3 nodes, 2 bits per axis
-// selecting {0: i*dx, 1: (i+1)*dx, 2: i*dx + dx/2}.  Built from the corner
-// tuples (GlobalFccGrid.cc:47-69) and the facet node triples
-// (MC_Domain.cc:42-47) by facetNodeCodes() on the host side of this file.
-__constant__ uint32_t c_facetCode[24];
-// The same codes in LDS, for the lookups indexed per lane (a facet chosen at
-// run time): an LDS read instead of a memory load on the tracking loop's
-// critical path.  Every kernel that tracks fills it (load_facet_codes, then a
-// barrier) before its first search.
+// The per-facet node codes (kFacetCodes below) in LDS, for the lookups
+// indexed per lane (a facet chosen at run time): an LDS read instead of a
+// memory load on the tracking loop's critical path.  Every kernel that
+// tracks fills it (load_facet_codes, then a barrier) before its first search.
 __shared__ uint32_t s_facetCode[24];
-__device__ __forceinline__ void load_facet_codes() {
-    if (threadIdx.x < 24) s_facetCode[threadIdx.x] = c_facetCode[threadIdx.x];
-}
 
 struct Cell9 {
     double X0, X1, Xh, Y0, Y1, Yh, Z0, Z1, Zh;
@@ -246,6 +236,29 @@ constexpr FacetTable makeFacetTable() {
 }
 
 constexpr FacetTable kFacet = makeFacetTable();
+
+// Per-facet node codes: for each of the facet's 3 nodes (node m at bit 6*m),
+// 2 bits per axis selecting {0: i*dx, 1: (i+1)*dx, 2: i*dx + dx/2}.  They are
+// compile-time constants, so a facet known at compile time costs no
+// register; per-lane lookups read the LDS copy.
+struct FacetCodes {
+    uint32_t c[24];
+};
+
+constexpr FacetCodes makeFacetCodes() {
+    FacetCodes k{};
+    for (int f = 0; f < 24; f++)
+        for (int m = 0; m < 3; m++)
+            k.c[f] |= (uint32_t)(kFacet.code[f][m][0] | (kFacet.code[f][m][1] << 2) | (kFacet.code[f][m][2] << 4))
+                      << (6 * m);
+    return k;
+}
+
+constexpr FacetCodes kFacetCodes = makeFacetCodes();
+
+__device__ __forceinline__ void load_facet_codes() {
+    if (threadIdx.x < 24) s_facetCode[threadIdx.x] = kFacetCodes.c[threadIdx.x];
+}
 
 // Which triangle of a face lies on which edge of the face.  Face (AX,
 // side) holds facets 8*AX + 4*side + T, T = 0..3; each is two corners plus
@@ -454,7 +467,7 @@ __device__ __forceinline__ void visit_facet(FindNearest& fn, const double* pl, c
         fn.add(kHuge, FM);
         return;
     }
-    const uint32_t code = plus ? c_facetCode[FP] : c_facetCode[FM];
+    const uint32_t code = plus ? kFacetCodes.c[FP] : kFacetCodes.c[FM];
     double N, D;
     plane_nd(AX, facet_node(c9, code, 0), facet_node(c9, code, 1), facet_node(c9, code, 2), N, D);
     const double dot = N * (AX == 0 ? a : (AX == 1 ? b : g));
@@ -491,7 +504,7 @@ __device__ __forceinline__ double distance_any_facet(const double* pl, const Cel
     const double A = p[0], B = p[1], C = p[2], D = p[3];
     const double dot = A * a + B * b + C * g;
     if (dot <= 0.0) return kHuge;
-    const uint32_t code = c_facetCode[f];
+    const uint32_t code = s_facetCode[f];
     const Node3 n0 = facet_node(c9, code, 0), n1 = facet_node(c9, code, 1), n2 = facet_node(c9, code, 2);
     switch (f >> 3) {
         case 0:
@@ -1638,7 +1651,7 @@ __device__ __forceinline__ void source_particle(const DevProblem& P, const Parti
     while (currentVolume < whichVolume) {
         facet++;
         if (facet == 24) break;
-        uint32_t code = c_facetCode[facet];
+        uint32_t code = s_facetCode[facet];
         p0 = facet_node(c9, code, 0);
         p1 = facet_node(c9, code, 1);
         p2 = facet_node(c9, code, 2);
@@ -1708,6 +1721,8 @@ __global__ void __launch_bounds__(256) k_source(DevProblem P, Particles out, con
                      (size_t)(gridDim.x - srcBlocks) * blockDim.x);
         return;
     }
+    load_facet_codes();
+    __syncthreads();
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nSource) return;
     source_particle(P, out, ctl->start + (uint32_t)t, t, cycle);
@@ -2309,6 +2324,8 @@ __device__ __forceinline__ int find_problem(const uint64_t* prefix, int n, uint6
 }
 
 __global__ void __launch_bounds__(256) k_batch_source(BatchSlot* slots, int n, const uint64_t* srcPrefix) {
+    load_facet_codes();
+    __syncthreads();
     const uint64_t total = srcPrefix[n];
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -2534,53 +2551,7 @@ __global__ void k_batch_flux_final(BatchSlot* slots, int n, double* fluxSums) {
     }
 }
 
-constexpr int kMaxDevices = 64;
-bool g_codesReady[kMaxDevices] = {};
-
-void facetNodeCodes(uint32_t* code) {
-    static const int corner[14][4] = {{0, 0, 0, 0}, {1, 0, 0, 0}, {0, 1, 0, 0}, {1, 1, 0, 0},
-                                      {0, 0, 1, 0}, {1, 0, 1, 0}, {0, 1, 1, 0}, {1, 1, 1, 0},
-                                      {1, 0, 0, 1}, {0, 0, 0, 1}, {0, 1, 0, 2}, {0, 0, 0, 2},
-                                      {0, 0, 1, 3}, {0, 0, 0, 3}};
-    static const int tri[24][3] = {{1, 3, 8},  {3, 7, 8},  {7, 5, 8},  {5, 1, 8},  {0, 4, 9},
-                                   {4, 6, 9},  {6, 2, 9},  {2, 0, 9},  {3, 2, 10}, {2, 6, 10},
-                                   {6, 7, 10}, {7, 3, 10}, {0, 1, 11}, {1, 5, 11}, {5, 4, 11},
-                                   {4, 0, 11}, {4, 5, 12}, {5, 7, 12}, {7, 6, 12}, {6, 4, 12},
-                                   {0, 2, 13}, {2, 3, 13}, {3, 1, 13}, {1, 0, 13}};
-    for (int f = 0; f < 24; f++) {
-        uint32_t c = 0;
-        for (int m = 0; m < 3; m++) {
-            const int* o = corner[tri[f][m]];
-            uint32_t xc = (o[3] == 2 || o[3] == 3) ? 2u : (uint32_t)o[0];
-            uint32_t yc = (o[3] == 1 || o[3] == 3) ? 2u : (uint32_t)o[1];
-            uint32_t zc = (o[3] == 1 || o[3] == 2) ? 2u : (uint32_t)o[2];
-            c |= (xc | (yc << 2) | (zc << 4)) << (6 * m);
-        }
-        code[f] = c;
-    }
-}
-
-// Uploads the facet node-code table to the current device once (per
-// device: each holds its own copy of constant memory).
-hipError_t ensureCodes(hipStream_t s) {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
-    static std::mutex m;  // problems may be built from several host threads
-    std::lock_guard<std::mutex> lock(m);
-    if (g_codesReady[dev]) return hipSuccess;
-    uint32_t code[24];
-    facetNodeCodes(code);
-    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_facetCode), code, sizeof(code), 0, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e == hipSuccess) g_codesReady[dev] = true;
-    return e;
-}
-
 }  // namespace
-
-int prepareDevice(const LaunchCtx& ctx) { return (int)ensureCodes((hipStream_t)ctx.stream); }
 
 size_t tableLdsBytes(const DevProblem& P) {
     return sizeof(double) * ((size_t)(P.G + 1) + 2 * (size_t)P.nMat * P.G + (size_t)P.nMat * P.maxR * P.G);

@@ -128,7 +128,6 @@ void build(qsg_problem_s* p, int device, uint64_t capacity) {
              "hipDeviceGetAttribute");
     for (hipEvent_t* e : {&p->ev0, &p->ev1, &p->evI0, &p->evI1, &p->evF0, &p->evF1})
         hipCheck(hipEventCreate(e), "hipEventCreate");
-    hipCheck((hipError_t)qs::prepareDevice(p->ctx()), "device set-up");
 
     const qs::ProblemTables& T = p->tables;
     if ((int)T.materials.size() > qs::kMaxMaterials)

@@ -195,7 +195,6 @@ qsg_batch_s* createBatch(qsg_problem* problems, int n, int capacity, int device,
                  "hipDeviceGetAttribute");
         hipCheck(hipEventCreate(&b->ev0), "hipEventCreate");
         hipCheck(hipEventCreate(&b->ev1), "hipEventCreate");
-        hipCheck((hipError_t)qs::prepareDevice(qs::LaunchCtx{(void*)b->stream, b->numCUs}), "device set-up");
         if (n > 0) b->tune = problems[0]->tune;
         const int c = capacity;
         b->probs.assign(c, nullptr);

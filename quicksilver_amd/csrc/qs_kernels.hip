@@ -42,16 +42,40 @@ constexpr uint32_t kMaxSegmentsPerHistory = 1u << 26;
 
 enum { kScatter = 1, kAbsorption = 2, kFission = 3 };
 enum : uint32_t {
-    kEvSegment = 1u, kEvCollision = 2u, kEvScatter = 4u, kEvAbsorb = 8u,
-    kEvFission = 16u, kEvEscape = 32u, kEvCensus = 64u
+    kEvCollision = 1u, kEvScatter = 2u, kEvAbsorb = 4u, kEvFission = 8u, kEvEscape = 16u, kEvCensus = 32u
 };
 enum { kEscape = 1, kReflect = 2 };
 
 // ---------------------------------------------------------------- RNG ----
 // 64-bit LCG (MC_RNG_State.hh:23-32).
+constexpr uint64_t kLcgMul = 2862933555777941757ULL, kLcgAdd = 3037000493ULL;
 __device__ __forceinline__ double rng_sample(uint64_t& s) {
-    s = 2862933555777941757ULL * s + 3037000493ULL;
+    s = kLcgMul * s + kLcgAdd;
     return 5.4210108624275222e-20 * (double)s;
+}
+
+// n steps of the LCG as one multiply-add: s_n = M s + C with M = mul^n and
+// C = add (mul^(n-1) + ... + mul + 1), both mod 2^64 (the standard jump-ahead
+// of a linear congruential generator).
+struct LcgJump {
+    uint64_t mul, add;
+};
+
+constexpr LcgJump lcgJump(int n) {
+    LcgJump j{1, 0};
+    for (int i = 0; i < n; i++) {
+        j.mul = j.mul * kLcgMul;
+        j.add = j.add * kLcgMul + kLcgAdd;
+    }
+    return j;
+}
+
+// The seed after m more (energy, angle) pairs, m = 1..3: 2m draws skipped.
+__device__ __forceinline__ uint64_t rng_skip_pairs(uint64_t s, int m) {
+    constexpr LcgJump j1 = lcgJump(2), j2 = lcgJump(4), j3 = lcgJump(6);
+    const uint64_t mul = m == 1 ? j1.mul : (m == 2 ? j2.mul : j3.mul);
+    const uint64_t add = m == 1 ? j1.add : (m == 2 ? j2.add : j3.add);
+    return mul * s + add;
 }
 
 // Numerical Recipes psdes hash with two rounds (MC_RNG_State.cc:27-98).
@@ -1180,8 +1204,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     bool upd = false, rec = false, grp = false, ld = false;
     Lane L = {};
     uint32_t histSeg = 0;
-    // wave-uniform (scalar) balance counters, updated from ballots at the
-    // converged end of each iteration
+    // wave-uniform (scalar) balance counters: segments from the lanes that
+    // start one, the others from ballots at the converged end of each
+    // iteration
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
     double fluxAcc = 0.0;  // this lane's scalar-flux contributions
 
@@ -1254,7 +1279,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         }
         // room for this iteration's pushes (uniform: decided before any lane diverges)
         const bool stackOk = top + kStackMargin <= scap;
-        if (__ballot(active) == 0) break;  // drained
+        const uint64_t segLanes = __ballot(active);  // each takes one segment below
+        if (segLanes == 0) break;                    // drained
+        cSeg += __popcll(segLanes);
         // ---- per-lane work of loaded particles and collision survivors,
         // one copy for all lanes that need it ----
         bool fresh = false;  // loaded particle starting without mean free paths
@@ -1354,7 +1381,6 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             unsafeAtomicAdd(&P.flux[(size_t)cell * G + L.group], contrib);
             fluxAcc += contrib;
         }
-        ev |= kEvSegment;
         histSeg++;
 
         if (outcome == 0) {
@@ -1407,15 +1433,15 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             // cur >= 0 after the full scan; the last reaction is used here.
             if (selR < 0) selR = mi.nReact - 1;
             const int type = T.reactType[mat * P.maxR + selR];
-            // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88).
-            // The outgoing (energy, angle) pairs are drawn here only to
-            // advance the seed; each output replays its pair from s0 below.
-            const uint64_t s0 = L.seed;
+            // NuclearDataReaction::sampleCollision (NuclearData.cc:54-88):
+            // the multiplicity, then the outgoing (energy, angle) pairs in
+            // order, each drawn once.  Pair 0 is the parent's.
             int nOut = 0;
+            double e0 = 0.0, a0 = 0.0;
             if (type == kScatter) {
                 nOut = 1;
-                rng_sample(L.seed);
-                rng_sample(L.seed);
+                e0 = L.E * (1.0 - (rng_sample(L.seed) * (1.0 / mi.mass)));
+                a0 = rng_sample(L.seed) * 2.0 - 1.0;
             } else if (type == kFission) {
                 int n = (int)(mi.nuBar + rng_sample(L.seed));
                 nOut = n;
@@ -1423,10 +1449,10 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                     atomicOr(io.error, 16u);  // MAX_PRODUCTION_SIZE exceeded
                     nOut = 4;
                 }
-#pragma unroll 1
-                for (int k = 0; k < nOut; k++) {
-                    rng_sample(L.seed);
-                    rng_sample(L.seed);
+                if (nOut > 0) {
+                    const double rr = rng_sample(L.seed) / 2.0 + 0.5;
+                    e0 = (20 * rr * rr);
+                    a0 = rng_sample(L.seed) * 2.0 - 1.0;
                 }
             }
             ev |= kEvCollision;
@@ -1439,52 +1465,40 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             if (nOut == 0) {
                 active = false;
             } else {
-                // Outputs: secondaries k = 1..nOut-1 are copies of the
-                // pre-collision parent with spawned seeds and outgoing pair k
+                // Outputs: secondaries j = 1..nOut-1 are copies of the
+                // pre-collision parent with outgoing pair j and spawned seeds
                 // (CollisionEvent.cc:125-133); the parent takes pair 0 and the
                 // seed left after the spawns (CollisionEvent.cc:135) and stays
                 // in the lane, as the reference's parent continues its
-                // history.  The secondaries go to the wave's stack (or the
-                // pass's extra queue when the stack is full) as records whose
+                // history.  The spawns start from the seed after the last
+                // pair, one jump of 2 (nOut - 1) draws from the seed after
+                // pair 0, while pairs 1.. are drawn from that seed in order.
+                // The secondaries go to the wave's stack (or the pass's extra
+                // queue when the stack is full) as records whose
                 // updateTrajectory runs when they are loaded.
-                const Particles dst = stackOk ? stack : io.extra;
-                const uint32_t dcap = stackOk ? scap : io.extraCap;
-                uint32_t slot = 0;
-                if (nOut > 1)
-                    slot = stackOk ? wave_push(sTopW, (uint32_t)(nOut - 1)) : wave_append(io.nExtra, (uint32_t)(nOut - 1));
+                if (nOut > 1) {
+                    const Particles dst = stackOk ? stack : io.extra;
+                    const uint32_t dcap = stackOk ? scap : io.extraCap;
+                    const uint32_t slot = stackOk ? wave_push(sTopW, (uint32_t)(nOut - 1))
+                                                  : wave_append(io.nExtra, (uint32_t)(nOut - 1));
+                    uint64_t t = L.seed;
+                    L.seed = rng_skip_pairs(t, nOut - 1);
 #pragma unroll 1
-                for (int k = 1; k <= nOut; k++) {
-                    const bool parent = (k == nOut);
-                    const uint64_t seedK = parent ? L.seed : rng_spawn(L.seed);
-                    const int j = parent ? 0 : k;
-                    uint64_t t = s0;
-                    double ek, ak;
-                    if (type == kScatter) {
-                        ek = L.E * (1.0 - (rng_sample(t) * (1.0 / mi.mass)));
-                        ak = rng_sample(t) * 2.0 - 1.0;
-                    } else {
-                        rng_sample(t);  // the multiplicity draw
-#pragma unroll 1
-                        for (int i = 0; i < j; i++) {
-                            rng_sample(t);
-                            rng_sample(t);
-                        }
-                        double rr = rng_sample(t) / 2.0 + 0.5;
-                        ek = (20 * rr * rr);
-                        ak = rng_sample(t) * 2.0 - 1.0;
-                    }
-                    if (parent) {
-                        // the parent: updated at the top of the next iteration
-                        L.E = ek;
-                        L.nmfp = ak;
-                        upd = true;
-                        grp = true;
-                    } else {
-                        const uint32_t o = slot + (uint32_t)(k - 1);
-                        if (o < dcap) store_deferred(dst, o, L, seedK, ek, ak, cell);
+                    for (int j = 1; j < nOut; j++) {
+                        const double rr = rng_sample(t) / 2.0 + 0.5;
+                        const double ek = (20 * rr * rr);
+                        const double ak = rng_sample(t) * 2.0 - 1.0;
+                        const uint64_t seedJ = rng_spawn(L.seed);
+                        const uint32_t o = slot + (uint32_t)(j - 1);
+                        if (o < dcap) store_deferred(dst, o, L, seedJ, ek, ak, cell);
                         else atomicOr(io.error, 2u);
                     }
                 }
+                // the parent: updated at the top of the next iteration
+                L.E = e0;
+                L.nmfp = a0;
+                upd = true;
+                grp = true;
             }
         } else if (outcome == 1) {
             // ---- MC_Facet_Crossing_Event (MC_Facet_Crossing_Event.cc:25-70) ----
@@ -1539,7 +1553,6 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             grp = false;
         }
         }  // active
-        cSeg += __popcll(__ballot(ev & kEvSegment));
         cCol += __popcll(__ballot(ev & kEvCollision));
         cScat += __popcll(__ballot(ev & kEvScatter));
         cAbs += __popcll(__ballot(ev & kEvAbsorb));

@@ -227,12 +227,6 @@ __device__ __forceinline__ void plane_nd(int ax, const Node3& r0, const Node3& r
     D = d * inv;
 }
 
-// N and D of facet f of the cell (see plane_nd).
-__device__ __forceinline__ void facet_nd(const Cell9& c9, int f, double& N, double& D) {
-    const uint32_t code = s_facetCode[f];
-    plane_nd(f >> 3, facet_node(c9, code, 0), facet_node(c9, code, 1), facet_node(c9, code, 2), N, D);
-}
-
 // Cell-local node triples of the 24 facets (MC_Domain.cc:42-47) resolved to
 // per-axis value codes of the corner tuples (GlobalFccGrid.cc:47-69):
 // 0 -> i*dx, 1 -> (i+1)*dx, 2 -> i*dx + dx/2.  kFacet.code[f][node][axis].
@@ -261,27 +255,108 @@ constexpr FacetTable makeFacetTable() {
 
 constexpr FacetTable kFacet = makeFacetTable();
 
-// Per-facet node codes: for each of the facet's 3 nodes (node m at bit 6*m),
-// 2 bits per axis selecting {0: i*dx, 1: (i+1)*dx, 2: i*dx + dx/2}.  They are
-// compile-time constants, so a facet known at compile time costs no
-// register; per-lane lookups read the LDS copy.
+// Per-facet codes.  Bits 0-17: for each of the facet's 3 nodes (node m at
+// bit 6*m), 2 bits per axis selecting {0: i*dx, 1: (i+1)*dx, 2: i*dx + dx/2}.
+// Bits 18-23 give the facet's plane directly (facet_nd): every facet is two
+// corners that share an edge of its face, then the face centre.  In
+// plane_nd's (p, q) axes, when the corners share p (p1 == p0), the first
+// product (p1 - p0)(q2 - q0) is a signed zero and the nonzero component is
+// c = -(q1 - q0)(Ph - p0); when they share q, the second product is a signed
+// zero and c = (p1 - p0)(Qh - q0).  Either way c = +-(E * H), with E the
+// cell's extent along the edge's axis (q1 - q0 or p1 - p0 is +-E exactly)
+// and H the face centre's offset from the edge across it, so:
+//   bits 18-19 the edge's axis (E = X1 - X0, Y1 - Y0 or Z1 - Z0),
+//   bits 20-21 the axis across the edge, bit 22 the edge's end of that axis
+//   (H = Xh - X0 or Xh - X1, ...), bit 23 set when c = -(E * H).
+// They are compile-time constants, so a facet known at compile time costs
+// no register; per-lane lookups read the LDS copy.
 struct FacetCodes {
     uint32_t c[24];
 };
 
 constexpr FacetCodes makeFacetCodes() {
     FacetCodes k{};
-    for (int f = 0; f < 24; f++)
+    for (int f = 0; f < 24; f++) {
         for (int m = 0; m < 3; m++)
             k.c[f] |= (uint32_t)(kFacet.code[f][m][0] | (kFacet.code[f][m][1] << 2) | (kFacet.code[f][m][2] << 4))
                       << (6 * m);
+        const int ax = f >> 3, p = (ax + 1) % 3, q = (ax + 2) % 3;
+        const int p0 = kFacet.code[f][0][p], p1 = kFacet.code[f][1][p];
+        const int q0 = kFacet.code[f][0][q], q1 = kFacet.code[f][1][q];
+        uint32_t eAx, hAx, hEnd, neg;
+        if (p0 == p1) {  // c = -(q1 - q0)(Ph - p0)
+            eAx = (uint32_t)q;
+            hAx = (uint32_t)p;
+            hEnd = (uint32_t)p0;
+            neg = q1 > q0 ? 1u : 0u;
+        } else {  // c = (p1 - p0)(Qh - q0)
+            eAx = (uint32_t)p;
+            hAx = (uint32_t)q;
+            hEnd = (uint32_t)q0;
+            neg = p1 < p0 ? 1u : 0u;
+        }
+        k.c[f] |= (eAx << 18) | (hAx << 20) | (hEnd << 22) | (neg << 23);
+    }
     return k;
 }
 
 constexpr FacetCodes kFacetCodes = makeFacetCodes();
 
+// The table's premises, checked at compile time: on every facet the face
+// axis holds one corner value (0 or 1) for all three nodes (on the + face
+// for facets 8*ax + 0..3, the - face for 8*ax + 4..7), the third node is the
+// face centre (code 2 on both other axes), and the two corners differ on
+// exactly one of the other axes, by one edge (codes 0 and 1).
+constexpr bool facetTableHolds() {
+    for (int f = 0; f < 24; f++) {
+        const int ax = f >> 3, p = (ax + 1) % 3, q = (ax + 2) % 3;
+        const int face = (f & 4) ? 0 : 1;
+        for (int m = 0; m < 3; m++)
+            if (kFacet.code[f][m][ax] != face) return false;
+        if (kFacet.code[f][2][p] != 2 || kFacet.code[f][2][q] != 2) return false;
+        const int p0 = kFacet.code[f][0][p], p1 = kFacet.code[f][1][p];
+        const int q0 = kFacet.code[f][0][q], q1 = kFacet.code[f][1][q];
+        if (p0 > 1 || p1 > 1 || q0 > 1 || q1 > 1) return false;
+        if ((p0 == p1) == (q0 == q1)) return false;
+    }
+    return true;
+}
+static_assert(facetTableHolds(), "facet node table");
+
 __device__ __forceinline__ void load_facet_codes() {
     if (threadIdx.x < 24) s_facetCode[threadIdx.x] = kFacetCodes.c[threadIdx.x];
+}
+
+// N and D of facet f of the cell: plane_nd's arithmetic on the facet's own
+// nodes, with c formed as +-(E * H) (see kFacetCodes; the same products and
+// differences of the same node values, so the same bits), for a facet
+// chosen per lane at run time.
+__device__ __forceinline__ double in_vgpr(double v) {
+    // the value as a register operand: keeps per-lane selects among a
+    // struct's fields from becoming an indexed load of a stack copy
+    asm("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ void facet_nd(const Cell9& c9, int f, double& N, double& D) {
+    const uint32_t code = s_facetCode[f];
+    const int ax = f >> 3;
+    const uint32_t eAx = (code >> 18) & 3u, hAx = (code >> 20) & 3u;
+    const bool hEnd = (code >> 22) & 1u, neg = (code >> 23) & 1u;
+    const double X0 = in_vgpr(c9.X0), X1 = in_vgpr(c9.X1), Xh = in_vgpr(c9.Xh);
+    const double Y0 = in_vgpr(c9.Y0), Y1 = in_vgpr(c9.Y1), Yh = in_vgpr(c9.Yh);
+    const double Z0 = in_vgpr(c9.Z0), Z1 = in_vgpr(c9.Z1), Zh = in_vgpr(c9.Zh);
+    const double E = eAx == 0 ? X1 - X0 : (eAx == 1 ? Y1 - Y0 : Z1 - Z0);
+    const double Hc = hAx == 0 ? Xh : (hAx == 1 ? Yh : Zh);
+    const double He = hAx == 0 ? (hEnd ? X1 : X0) : (hAx == 1 ? (hEnd ? Y1 : Y0) : (hEnd ? Z1 : Z0));
+    double c = E * (Hc - He);
+    if (neg) c = -c;
+    const bool minus = (f & 4) != 0;
+    const double x0 = ax == 0 ? (minus ? X0 : X1) : (ax == 1 ? (minus ? Y0 : Y1) : (minus ? Z0 : Z1));
+    const double d = -1.0 * (c * x0);
+    const double inv = 1.0 / sqrt(c * c);
+    N = c * inv;
+    D = d * inv;
 }
 
 // Which triangle of a face lies on which edge of the face.  Face (AX,

@@ -212,6 +212,17 @@ __device__ __forceinline__ Node3 facet_node(const Cell9& c, uint32_t code, int m
 // plane sum is recomputed from the stored four-term plane
 // (facet_plane_distance, plane_sum).  No memory is read: the nodes come
 // from the cell's nine coordinates.
+// 1.0 / sqrt(c * c), the normalisation of a plane whose normal has one
+// nonzero component c.  In binary floating point with rounding to nearest
+// the correctly rounded square root of fl(c * c) is |c| whenever c * c is a
+// normal number, so the root is taken only outside that range.
+__device__ __forceinline__ double inv_norm(double c) {
+    const double cc = c * c;
+    double r = fabs(c);
+    if (!(cc >= 0x1p-1000 && cc <= 0x1p1000)) r = sqrt(cc);
+    return 1.0 / r;
+}
+
 __device__ __forceinline__ void plane_nd(int ax, const Node3& r0, const Node3& r1, const Node3& r2, double& N,
                                          double& D) {
     // (p, q) = the axes after ax in cyclic order: the nonzero component is
@@ -354,7 +365,7 @@ __device__ __forceinline__ void facet_nd(const Cell9& c9, int f, double& N, doub
     const bool minus = (f & 4) != 0;
     const double x0 = ax == 0 ? (minus ? X0 : X1) : (ax == 1 ? (minus ? Y0 : Y1) : (minus ? Z0 : Z1));
     const double d = -1.0 * (c * x0);
-    const double inv = 1.0 / sqrt(c * c);
+    const double inv = inv_norm(c);
     N = c * inv;
     D = d * inv;
 }
@@ -788,38 +799,53 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
     Nearest nf;
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
-        FindNearest fn = {1e80, -1, -1, -kHuge, 0};
         const FaceProbe f0 = probe_exit_face<0>(c9, x, y, z, a, b, g);
         const FaceProbe f1 = probe_exit_face<1>(c9, x, y, z, a, b, g);
         const FaceProbe f2 = probe_exit_face<2>(c9, x, y, z, a, b, g);
         // the lane's candidates in facet order (the faces are probed in
-        // facet order, each face's candidates ascending), at most three
-        int n = 0, q0 = -1, q1 = -1, q2 = -1;
-        const int cs[6] = {f0.c1, f0.c2, f1.c1, f1.c2, f2.c1, f2.c2};
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            if (cs[k] >= 0) {
-                if (n == 0) q0 = cs[k];
-                else if (n == 1) q1 = cs[k];
-                else if (n == 2) q2 = cs[k];
-                n++;
-            }
-        }
+        // facet order, each face's candidates ascending; a face's c2 is set
+        // only with its c1): their number and the first
+        const int n = (f0.c1 >= 0) + (f0.c2 >= 0) + (f1.c1 >= 0) + (f1.c2 >= 0) + (f2.c1 >= 0) + (f2.c2 >= 0);
+        const int q0 = f0.c1 >= 0 ? f0.c1 : (f1.c1 >= 0 ? f1.c1 : f2.c1);
         const bool unsure = f0.unsure || f1.unsure || f2.unsure;
         const bool deep = !unsure && n == 1 && (f0.c1 >= 0 ? f0.deep : (f1.c1 >= 0 ? f1.deep : f2.deep));
         if (deep) {
             // the only candidate, its exit point deeper inside it than
             // margin: the reference's containment test passes for it, so
-            // only the distance is computed
+            // only the distance is computed.  FindNearest over {q0: d, every
+            // other facet kHuge} then gives (d, q0) for -kHuge < d < kHuge
+            // (a positive d as the nearest, a non-positive one as the
+            // largest non-positive); otherwise kHuge at the largest facet
+            // index whose distance is kHuge (23, or 22 when q0 is 23 and its
+            // own d is not kHuge).
             const double d = facet_plane_distance(q0, c9, pl, planeTol, x, y, z, a, b, g);
-            fn.add(d, q0);
-            fn.add(kHuge, (d != kHuge && q0 == 23) ? 22 : 23);
+            if (d > -kHuge && d < kHuge) {
+                nf.d = d;
+                nf.f = q0;
+            } else {
+                nf.d = kHuge;
+                nf.f = (d != kHuge && q0 == 23) ? 22 : 23;
+            }
         } else if (unsure || n > 3) {
             // the reference's test on the 12 facets of the three exit faces
+            FindNearest fn = {1e80, -1, -1, -kHuge, 0};
             visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
             visit_axis<1>(fn, pl, c9, planeTol, x, y, z, a, b, g);
             visit_axis<2>(fn, pl, c9, planeTol, x, y, z, a, b, g);
+            nf = fn.result();
         } else {
+            FindNearest fn = {1e80, -1, -1, -kHuge, 0};
+            // the second and third candidates in facet order (n <= 3 here)
+            int m = 0, q1 = -1, q2 = -1;
+            const int cs[6] = {f0.c1, f0.c2, f1.c1, f1.c2, f2.c1, f2.c2};
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                if (cs[k] >= 0) {
+                    if (m == 1) q1 = cs[k];
+                    else if (m == 2) q2 = cs[k];
+                    m++;
+                }
+            }
             // exact tests of the candidates, merged in facet order; every
             // other facet is kHuge, which only sets the largest kHuge index
             bool fin0 = false, fin1 = false, fin2 = false;
@@ -838,8 +864,8 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
             for (int r = 0; r < 3; r++)
                 if ((fin0 && q0 == h) || (fin1 && q1 == h) || (fin2 && q2 == h)) h--;
             fn.add(kHuge, h);
+            nf = fn.result();
         }
-        nf = fn.result();
     }
     if (nf.d == kHuge) {
         int iteration = 0;

@@ -1500,12 +1500,40 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 const double m1 = mi.nReact > 1 ? mrow[G] : 0.0;
                 const double m2 = mi.nReact > 2 ? mrow[2 * G] : 0.0;
                 const bool has1 = mi.nReact > 1, has2 = mi.nReact > 2;
-                // One isotope per step as straight-line code: the same
-                // subtractions in the same order, the first partial sum
-                // below zero selecting the reaction; a lane that has
-                // selected, or scanned all its isotopes, keeps its values
-                // while the wave finishes the others (a uniform loop, with
-                // no per-lane branch to leave it).
+                // The scan's answer without the scan.  Exactly, the partial
+                // sums are cur - k S - M_r (isotope k, M_r = m0 + .. + m_r,
+                // S = M_(nReact-1)), decreasing, so the first negative one
+                // lies in isotope k = floor(cur / S), at the first r with
+                // rem = cur - k S < M_r.  The scan's rounded sums differ
+                // from the exact ones by at most one rounding per
+                // subtraction, under 3 nIso 2^-53 (cur + S), and the
+                // estimates here add a few more roundings; when the sums on
+                // both sides of the crossing are farther from zero than the
+                // bound B (four times that), the rounded scan changes sign
+                // at the same place and selects the same reaction.  A lane
+                // whose crossing is nearer zero, or none within its
+                // isotopes, runs the scan below.
+                {
+                    const double M1 = m0 + m1, S = M1 + m2;
+                    if (S > 0.0) {
+                        const double kq = floor(cur / S);
+                        if (kq < (double)mi.nIso) {
+                            const double rem = fma(-kq, S, cur);
+                            const double B = (12.0 * mi.nIso + 16.0) * 0x1p-53 * (cur + S);
+                            const int r = rem < m0 ? 0 : (rem < M1 ? 1 : 2);
+                            const double below = r == 0 ? 0.0 : (r == 1 ? m0 : M1);  // M_(r-1)
+                            const double at = r == 0 ? m0 : (r == 1 ? M1 : S);        // M_r
+                            if (r < mi.nReact && rem - below >= B && rem - at <= -B) selR = r;
+                        }
+                    }
+                }
+                // The scan itself, for the lanes left: one isotope per step
+                // as straight-line code, the same subtractions in the same
+                // order, the first partial sum below zero selecting the
+                // reaction; a lane that has selected, or scanned all its
+                // isotopes, keeps its values while the wave finishes the
+                // others (a uniform loop, with no per-lane branch to leave
+                // it).
                 for (int iso = 0;; iso++) {
                     const bool live = selR < 0 && iso < mi.nIso;
                     if (__ballot(live) == 0) break;

@@ -952,19 +952,28 @@ __device__ __forceinline__ double vlength(double x, double y, double z) {
     return sqrt(x * x + y * y + z * z);  // MC_Vector::Length
 }
 
+// Writes record i of o.  The fields lie one stride apart (Particles): one
+// address stepped by the stride per field, a single 64-bit add each, so no
+// per-field base address is held in registers.
+__device__ __forceinline__ void store_record(const Particles& o, uint32_t i, double x, double y, double z,
+                                             double v0, double v1, double v2, double e, double w, double tc,
+                                             double nmfp, uint64_t seed, int32_t cell) {
+    typedef __attribute__((address_space(1))) char* Bytes;
+    Bytes fp = (Bytes)o.base + (uint64_t)i * 8;
+    const double v[10] = {x, y, z, v0, v1, v2, e, w, tc, nmfp};
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        *(__attribute__((address_space(1))) double*)fp = v[k];
+        fp += o.stride;
+        asm("" : "+v"(fp));  // keep the step an add
+    }
+    *(__attribute__((address_space(1))) uint64_t*)fp = seed;
+    fp += o.stride;
+    *(__attribute__((address_space(1))) int32_t*)(fp - (uint64_t)i * 4) = cell;
+}
+
 __device__ __forceinline__ void store_particle(const Particles& o, uint32_t i, const Lane& L, int cell) {
-    o.x()[i] = L.x;
-    o.y()[i] = L.y;
-    o.z()[i] = L.z;
-    o.vx()[i] = L.vx;
-    o.vy()[i] = L.vy;
-    o.vz()[i] = L.vz;
-    o.E()[i] = L.E;
-    o.w()[i] = L.w;
-    o.tc()[i] = L.tc;
-    o.nmfp()[i] = L.nmfp;
-    o.seed()[i] = L.seed;
-    o.cell()[i] = cell;
+    store_record(o, i, L.x, L.y, L.z, L.vx, L.vy, L.vz, L.E, L.w, L.tc, L.nmfp, L.seed, cell);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -983,18 +992,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // the particle is the same as if it had been updated before the store.
 __device__ __forceinline__ void store_deferred(const Particles& o, uint32_t i, const Lane& L, uint64_t seed,
                                                double energy, double angle, int cell) {
-    o.x()[i] = L.x;
-    o.y()[i] = L.y;
-    o.z()[i] = L.z;
-    o.vx()[i] = L.a;
-    o.vy()[i] = L.b;
-    o.vz()[i] = L.g;
-    o.E()[i] = energy;
-    o.w()[i] = L.w;
-    o.tc()[i] = L.tc;
-    o.nmfp()[i] = angle;
-    o.seed()[i] = seed;
-    o.cell()[i] = ~cell;
+    store_record(o, i, L.x, L.y, L.z, L.a, L.b, L.g, energy, L.w, L.tc, angle, seed, ~cell);
 }
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t count) {
     uint64_t act = __ballot(1);
@@ -1276,6 +1274,10 @@ __device__ __forceinline__ void flush_tallies(DevCounters* dc, uint32_t& cSeg, u
 // Per-wave secondary stack: a view into the launch's stack memory and its
 // top, kept in LDS so the lane that reserves entries inside divergent code
 // can publish the new top to the rest of the wave.
+// A byte address in global memory (address space 1): loads through it stay
+// global loads after an optimisation barrier.
+typedef const __attribute__((address_space(1))) char* GlobalBytes;
+
 struct StackRef {
     Particles view;
     uint32_t cap;
@@ -1357,20 +1359,23 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 const bool fromStack = rank < takeS;
                 const uint32_t r = rank - takeS;
                 const uint32_t idx = fromStack ? top - 1 - rank : (r < aN ? aBase + r : bBase + (r - aN));
-                const Particles src = {fromStack ? stack.base : io.in.base,
-                                       fromStack ? stack.stride : io.in.stride};
-                L.x = src.x()[idx];
-                L.y = src.y()[idx];
-                L.z = src.z()[idx];
-                L.vx = src.vx()[idx];
-                L.vy = src.vy()[idx];
-                L.vz = src.vz()[idx];
-                L.E = src.E()[idx];
-                L.w = src.w()[idx];
-                L.tc = src.tc()[idx];
-                L.nmfp = src.nmfp()[idx];
-                L.seed = src.seed()[idx];
-                L.ci = src.cell()[idx];  // decoded after the refill
+                // The record's fields lie one stride apart (Particles): one
+                // address stepped by the lane's stride per field, a single
+                // 64-bit add each, rather than base + k * stride formed per
+                // field with the stride chosen per lane.
+                const uint64_t stride = fromStack ? stack.stride : io.in.stride;
+                GlobalBytes fp = (GlobalBytes)(fromStack ? stack.base : io.in.base) + (uint64_t)idx * 8;
+                double* const dst[10] = {&L.x, &L.y, &L.z, &L.vx, &L.vy, &L.vz, &L.E, &L.w, &L.tc, &L.nmfp};
+#pragma unroll
+                for (int k = 0; k < 10; k++) {
+                    *dst[k] = *(const __attribute__((address_space(1))) double*)fp;
+                    fp += stride;
+                    asm("" : "+v"(fp));  // keep the step an add
+                }
+                L.seed = *(const __attribute__((address_space(1))) uint64_t*)fp;
+                fp += stride;
+                // the int32 cell field, decoded after the refill
+                L.ci = *(const __attribute__((address_space(1))) int32_t*)(fp - (uint64_t)idx * 4);
                 ld = true;
                 histSeg = 0;
                 active = true;

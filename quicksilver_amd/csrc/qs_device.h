@@ -53,6 +53,7 @@ struct MatInfo {
     int nReact;
     double mass;
     double nuBar;
+    double invMass;  // 1.0 / mass, the scatter's energy factor (NuclearData.cc:64)
 };
 
 constexpr int kMaxMaterials = 16;

@@ -1488,6 +1488,10 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             fluxAcc += contrib;
         }
         histSeg++;
+        // a census particle's slot, reserved before the collision and
+        // crossing work so that the append's round trip passes under it
+        uint32_t censusSlot = 0;
+        if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);
 
         if (outcome == 0) {
             // ---- CollisionEvent (CollisionEvent.cc:50-148) ----
@@ -1498,7 +1502,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             // Every isotope of a material carries the same per-reaction
             // macroscopic cross sections, so the scan subtracts the same
             // nReact values once per isotope; with up to three reactions
-            // they are read once and held in registers.
+            // they are the ones read with sigt.
             const double* mrow = T.macro + (size_t)(mat * P.maxR) * G + L.group;
             if (mi.nReact <= 3) {
                 const double m0 = mrow[0];
@@ -1574,7 +1578,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             double e0 = 0.0, a0 = 0.0;
             if (type == kScatter) {
                 nOut = 1;
-                e0 = L.E * (1.0 - (rng_sample(L.seed) * (1.0 / mi.mass)));
+                e0 = L.E * (1.0 - (rng_sample(L.seed) * mi.invMass));  // 1.0 / mass
                 a0 = rng_sample(L.seed) * 2.0 - 1.0;
             } else if (type == kFission) {
                 int n = (int)(mi.nuBar + rng_sample(L.seed));
@@ -1670,7 +1674,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             }
         } else {
             // ---- census: processedVault->pushParticle (CycleTracking.cc:104-111) ----
-            uint32_t o = wave_append(io.nCensus, 1u);
+            const uint32_t o = censusSlot;
             if (o < io.censusCap) store_particle(io.census, o, L, cell);
             else atomicOr(io.error, 1u);
             // EnergySpectrum::UpdateSpectrum (EnergySpectrum.cc:12-35, called

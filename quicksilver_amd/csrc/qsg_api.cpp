@@ -187,6 +187,7 @@ void build(qsg_problem_s* p, int device, uint64_t capacity) {
         d.mat[m].nIso = mt.nIsotopes;
         d.mat[m].nReact = mt.nReactions;
         d.mat[m].mass = mt.mass;
+        d.mat[m].invMass = 1.0 / mt.mass;
         d.mat[m].nuBar = mt.nuBar;
     }
     d.total = p->upload(total);

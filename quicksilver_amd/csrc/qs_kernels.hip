@@ -279,6 +279,12 @@ constexpr FacetTable kFacet = makeFacetTable();
 //   bits 18-19 the edge's axis (E = X1 - X0, Y1 - Y0 or Z1 - Z0),
 //   bits 20-21 the axis across the edge, bit 22 the edge's end of that axis
 //   (H = Xh - X0 or Xh - X1, ...), bit 23 set when c = -(E * H).
+// Bits 24-26: the sign bits of the plane's x, y and z components where they
+// are zero (the two other than the face axis).  MC_General_Plane
+// (MC_Facet_Geometry.hh:18-39) forms them from products with a difference
+// of equal face coordinates, +0, so each is a signed zero whose sign follows
+// from the signs of the node differences alone: evaluated here on nodes
+// 1 < centre 2 < 3 along every axis, the same signs as on any cell.
 // They are compile-time constants, so a facet known at compile time costs
 // no register; per-lane lookups read the LDS copy.
 struct FacetCodes {
@@ -307,6 +313,19 @@ constexpr FacetCodes makeFacetCodes() {
             neg = p1 < p0 ? 1u : 0u;
         }
         k.c[f] |= (eAx << 18) | (hAx << 20) | (hEnd << 22) | (neg << 23);
+        // MC_General_Plane's A, B, C on representative nodes
+        double r[3][3];
+        for (int m = 0; m < 3; m++)
+            for (int a = 0; a < 3; a++) {
+                const int cd = kFacet.code[f][m][a];
+                r[m][a] = cd == 0 ? 1.0 : (cd == 1 ? 3.0 : 2.0);
+            }
+        const double comp[3] = {
+            ((r[1][1] - r[0][1]) * (r[2][2] - r[0][2])) - ((r[1][2] - r[0][2]) * (r[2][1] - r[0][1])),
+            ((r[1][2] - r[0][2]) * (r[2][0] - r[0][0])) - ((r[1][0] - r[0][0]) * (r[2][2] - r[0][2])),
+            ((r[1][0] - r[0][0]) * (r[2][1] - r[0][1])) - ((r[1][1] - r[0][1]) * (r[2][0] - r[0][0]))};
+        for (int a = 0; a < 3; a++)
+            if (a != ax && (__builtin_bit_cast(uint64_t, comp[a]) >> 63)) k.c[f] |= 1u << (24 + a);
     }
     return k;
 }
@@ -1657,9 +1676,17 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 ev |= kEvEscape;
                 active = false;
             } else {
-                // MCT_Reflect_Particle (MCT.cc:401-429)
-                const double* p = pl + 4 * nf.f;
-                double A = p[0], B = p[1], C = p[2];
+                // MCT_Reflect_Particle (MCT.cc:401-429) with the facet's
+                // plane normal formed in registers: N along the face axis
+                // (facet_nd, the stored plane's own arithmetic) and signed
+                // zeros elsewhere (kFacetCodes bits 24-26)
+                double N, Dn;
+                facet_nd(cell_nodes(P, L.ci, L.cj, L.ck), nf.f, N, Dn);
+                const uint32_t fcode = s_facetCode[nf.f];
+                const int fax = nf.f >> 3;
+                const double A = fax == 0 ? N : ((fcode >> 24) & 1u ? -0.0 : 0.0);
+                const double B = fax == 1 ? N : ((fcode >> 25) & 1u ? -0.0 : 0.0);
+                const double C = fax == 2 ? N : ((fcode >> 26) & 1u ? -0.0 : 0.0);
                 double dot = 2.0 * (L.a * A + L.b * B + L.g * C);
                 if (dot > 0) {
                     L.a -= dot * A;

@@ -12,7 +12,9 @@ component as +-(E * H): E the cell's extent along the shared edge, H the face
 centre's offset from that edge.  These tests take random cells, evaluate the
 reference's full cross product in IEEE double (numpy, no contraction), and
 require the kernels' form to give the same bits; they also require that
-sqrt(fl(c * c)) is |c|, which lets the kernels skip the square root.
+sqrt(fl(c * c)) is |c|, which lets the kernels skip the square root, and
+that the two other components are signed zeros whose signs depend only on
+the facet, which lets a reflection form the plane without reading it.
 """
 import numpy as np
 import pytest
@@ -98,3 +100,39 @@ def test_square_root_of_square_is_magnitude():
     e = rng.integers(-480, 480, m.size)
     c = np.ldexp(m, e) * np.where(rng.random(m.size) < 0.5, -1.0, 1.0)
     assert np.array_equal(np.sqrt(c * c), np.abs(c))
+
+
+def reference_plane_components(r0, r1, r2):
+    """MC_General_Plane's A, B, C before normalisation (MC_Facet_Geometry.hh:18-39)."""
+    A = ((r1[1] - r0[1]) * (r2[2] - r0[2])) - ((r1[2] - r0[2]) * (r2[1] - r0[1]))
+    B = ((r1[2] - r0[2]) * (r2[0] - r0[0])) - ((r1[0] - r0[0]) * (r2[2] - r0[2]))
+    C = ((r1[0] - r0[0]) * (r2[1] - r0[1])) - ((r1[1] - r0[1]) * (r2[0] - r0[0]))
+    return A, B, C
+
+
+@pytest.mark.parametrize("f", range(24))
+def test_zero_plane_components_have_signs_fixed_by_the_facet(f):
+    """The two plane components off the face axis are signed zeros whose
+    signs depend only on the facet: the same on random cells as on the
+    representative nodes 1 < 2 < 3 the kernels' table is built from (a
+    reflection reads them; MCT.cc:401-429)."""
+    codes = node_codes(f)
+    ax = f >> 3
+    rep = [[(1.0, 3.0, 2.0)[codes[m][a]] for a in range(3)] for m in range(3)]
+    rep_comp = reference_plane_components(*[np.array(rep[m]) for m in range(3)])
+    want = [np.signbit(rep_comp[a]) for a in range(3)]
+    rng = np.random.default_rng(300 + f)
+    vals = cell_values(rng, 20000)
+    nodes = [[pick(vals, a, codes[m][a]) for a in range(3)] for m in range(3)]
+    comp = reference_plane_components(*nodes)
+    for a in range(3):
+        if a == ax:
+            assert np.all(comp[a] != 0.0)
+            continue
+        assert np.all(comp[a] == 0.0)
+        assert np.all(np.signbit(comp[a]) == want[a]), (f, a)
+    # normalising by a positive factor keeps each sign
+    inv = 1.0 / np.sqrt(comp[0] * comp[0] + comp[1] * comp[1] + comp[2] * comp[2])
+    for a in range(3):
+        if a != ax:
+            assert np.all(np.signbit(comp[a] * inv) == want[a])

@@ -666,16 +666,31 @@ struct FaceProbe {
     bool deep;
 };
 
+// What a lane computes for each face of axis AX that its direction points
+// out of: the probe point relative to the face centre, the face's half
+// widths and margin, and whether the point lies on the face widened by
+// margin.  Only the face that holds the point gets the rest of the probe
+// (probe_face), once per lane whatever its axis.
+struct FaceLook {
+    double du, dv;  // probe point minus the face centre along (u, v), times dot0
+    double hu, hv;  // the face's half widths
+    double margin;
+    double dot0;    // |direction along the axis| > 0
+    bool plus;      // the + face of the axis
+    bool inside;    // on the face widened by margin
+    bool unsure;    // the probe cannot decide (see FaceProbe)
+};
+
 template <int AX>
-__device__ __forceinline__ FaceProbe probe_exit_face(const Cell9& c9, double x, double y,
-                                                     double z, double a, double b, double g) {
+__device__ __forceinline__ FaceLook look_exit_face(const Cell9& c9, double x, double y, double z, double a,
+                                                   double b, double g) {
     constexpr int U = AX == 0 ? 1 : (AX == 1 ? 2 : 0);
     constexpr int V = AX == 0 ? 2 : (AX == 1 ? 0 : 1);
-    FaceProbe r = {-1, -1, false, false};
+    FaceLook r = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false, false, false};
     const double dAx = AX == 0 ? a : (AX == 1 ? b : g);
     const bool plus = dAx > 0.0, minus = dAx < 0.0;
     if (!(plus || minus)) return r;  // parallel to both faces: dot = 0, all kHuge
-    const int base = 8 * AX + (plus ? 0 : 4);
+    r.plus = plus;
     // The face is the plane coordinate = Xf; the stored planes of its four
     // triangles equal it to the last bits, so the probe needs no plane loads.
     // dot0 > 0 and num0 are the direction and distance along the axis.
@@ -703,40 +718,52 @@ __device__ __forceinline__ FaceProbe probe_exit_face(const Cell9& c9, double x, 
     // probe point relative to the face centre, scaled by dot0 > 0 so that no
     // division is needed: du = (u - Uh) + (num0 / dot0) * dir_u, times dot0
     // (rounding here is ~1e-16 of quantities the margin exceeds by 1e6)
-    const double du = ((U == 0 ? x : (U == 1 ? y : z)) - Uh) * dot0 + num0 * (U == 0 ? a : (U == 1 ? b : g));
-    const double dv = ((V == 0 ? x : (V == 1 ? y : z)) - Vh) * dot0 + num0 * (V == 0 ? a : (V == 1 ? b : g));
-    // The face's diagonals split it into the four triangles (the one on the
-    // u = U1 edge is the wedge s1 > 0, s2 > 0 below, and so on).  A triangle
-    // across a diagonal lies beyond that diagonal's line, so it is a
-    // candidate only if the point is within `margin` of the line
-    // (|s| <= margin * (hu + hv) >= margin * |diagonal direction|).  Within
-    // margin of both diagonals -- the centre, where all four meet -- the
-    // probe decides nothing.  Lengths below are scaled by dot0 like du, dv.
-    const double s1 = du * hv - dv * hu, s2 = du * hv + dv * hu;
-    const double thr = margin * (hu + hv) * dot0;
+    r.du = ((U == 0 ? x : (U == 1 ? y : z)) - Uh) * dot0 + num0 * (U == 0 ? a : (U == 1 ? b : g));
+    r.dv = ((V == 0 ? x : (V == 1 ? y : z)) - Vh) * dot0 + num0 * (V == 0 ? a : (V == 1 ? b : g));
+    r.hu = hu;
+    r.hv = hv;
+    r.margin = margin;
+    r.dot0 = dot0;
+    // outside the face widened by margin, the reference's containment test
+    // fails for all four of its triangles (its tolerances are 1e-9)
+    r.inside = fabs(r.du) <= (hu + margin) * dot0 && fabs(r.dv) <= (hv + margin) * dot0;
+    return r;
+}
+
+// The rest of the probe for the face of axis ax that holds the point.  The
+// face's diagonals split it into the four triangles (the one on the u = U1
+// edge is the wedge s1 > 0, s2 > 0 below, and so on).  A triangle across a
+// diagonal lies beyond that diagonal's line, so it is a candidate only if the
+// point is within `margin` of the line (|s| <= margin * (hu + hv) >= margin *
+// |diagonal direction|).  Within margin of both diagonals -- the centre,
+// where all four meet -- the probe decides nothing.  Lengths are scaled by
+// dot0 like du, dv.
+__device__ __forceinline__ FaceProbe probe_face(int ax, const FaceLook& l) {
+    FaceProbe r = {-1, -1, false, false};
+    const double s1 = l.du * l.hv - l.dv * l.hu, s2 = l.du * l.hv + l.dv * l.hu;
+    const double thr = l.margin * (l.hu + l.hv) * l.dot0;
     const bool near1 = fabs(s1) <= thr, near2 = fabs(s2) <= thr;
     if (near1 && near2) {
         r.unsure = true;
         return r;
     }
-    if (fabs(du) <= (hu + margin) * dot0 && fabs(dv) <= (hv + margin) * dot0) {
-        const int side = plus ? 0 : 1;
-        // edge numbering of the edge table: 0 u = U0, 1 u = U1, 2 v = V0, 3 v = V1
-        const int wedge = s1 > 0.0 ? (s2 > 0.0 ? 1 : 2) : (s2 > 0.0 ? 3 : 0);
-        int other = -1;
-        if (near1) other = s2 > 0.0 ? (wedge == 1 ? 3 : 1) : (wedge == 2 ? 0 : 2);
-        else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
-        constexpr uint32_t kTri = edgeTriPacked<AX>();
-        const int t1 = (int)((kTri >> (2 * (4 * side + wedge))) & 3u);
-        r.c1 = base + t1;
-        // no second candidate: farther than margin from both diagonals;
-        // and farther than margin inside the face's own edges
-        r.deep = other < 0 && fabs(du) <= (hu - margin) * dot0 && fabs(dv) <= (hv - margin) * dot0;
-        if (other >= 0) {
-            const int t2 = (int)((kTri >> (2 * (4 * side + other))) & 3u);
-            r.c1 = base + (t1 < t2 ? t1 : t2);
-            r.c2 = base + (t1 < t2 ? t2 : t1);
-        }
+    const int side = l.plus ? 0 : 1;
+    const int base = 8 * ax + 4 * side;
+    // edge numbering of the edge table: 0 u = U0, 1 u = U1, 2 v = V0, 3 v = V1
+    const int wedge = s1 > 0.0 ? (s2 > 0.0 ? 1 : 2) : (s2 > 0.0 ? 3 : 0);
+    int other = -1;
+    if (near1) other = s2 > 0.0 ? (wedge == 1 ? 3 : 1) : (wedge == 2 ? 0 : 2);
+    else if (near2) other = s1 > 0.0 ? (wedge == 1 ? 2 : 1) : (wedge == 3 ? 0 : 3);
+    const uint32_t kTri = ax == 0 ? edgeTriPacked<0>() : (ax == 1 ? edgeTriPacked<1>() : edgeTriPacked<2>());
+    const int t1 = (int)((kTri >> (2 * (4 * side + wedge))) & 3u);
+    r.c1 = base + t1;
+    // no second candidate: farther than margin from both diagonals; and
+    // farther than margin inside the face's own edges
+    r.deep = other < 0 && fabs(l.du) <= (l.hu - l.margin) * l.dot0 && fabs(l.dv) <= (l.hv - l.margin) * l.dot0;
+    if (other >= 0) {
+        const int t2 = (int)((kTri >> (2 * (4 * side + other))) & 3u);
+        r.c1 = base + (t1 < t2 ? t1 : t2);
+        r.c2 = base + (t1 < t2 ? t2 : t1);
     }
     return r;
 }
@@ -807,27 +834,63 @@ __device__ __forceinline__ double exact_facet_test(int f, const double* pl, cons
 // facet bounds the particle it is nudged toward the cell centre and the
 // search repeats.  The first search considers only the three faces the
 // direction points out of (the other 12 facets have dot <= 0 and distance
-// kHuge in the reference too).  A probe of each of those faces names the one
-// or two triangles that can hold the exit point; the lane's candidates (at
-// most three) get the reference's exact test in facet order, and every
-// other facet is kHuge.  A lane whose probe cannot decide runs the exact
-// test on all 12 exit-face facets.  The rare retries use a compact loop
-// over all 24 facets so they add no register pressure to the common path.
+// kHuge in the reference too).  A look at each of those faces finds the one
+// that holds the exit point; the probe of that face names the one or two
+// triangles that can hold it; those candidates get the reference's exact
+// test in facet order, and every other facet is kHuge.  A lane whose probe
+// cannot decide, or whose point lies on two faces (within margin of an
+// edge), runs the exact test on all 12 exit-face facets.  The rare retries
+// use a compact loop over all 24 facets so they add no register pressure to
+// the common path.
 __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const double* pl, int cell, const Cell9& c9, double& x, double& y, double& z,
                                                  double a, double b, double g) {
     Nearest nf;
     {
         const double planeTol = 1e-16 * (x * x + y * y + z * z);
-        const FaceProbe f0 = probe_exit_face<0>(c9, x, y, z, a, b, g);
-        const FaceProbe f1 = probe_exit_face<1>(c9, x, y, z, a, b, g);
-        const FaceProbe f2 = probe_exit_face<2>(c9, x, y, z, a, b, g);
-        // the lane's candidates in facet order (the faces are probed in
-        // facet order, each face's candidates ascending; a face's c2 is set
-        // only with its c1): their number and the first
-        const int n = (f0.c1 >= 0) + (f0.c2 >= 0) + (f1.c1 >= 0) + (f1.c2 >= 0) + (f2.c1 >= 0) + (f2.c2 >= 0);
-        const int q0 = f0.c1 >= 0 ? f0.c1 : (f1.c1 >= 0 ? f1.c1 : f2.c1);
-        const bool unsure = f0.unsure || f1.unsure || f2.unsure;
-        const bool deep = !unsure && n == 1 && (f0.c1 >= 0 ? f0.deep : (f1.c1 >= 0 ? f1.deep : f2.deep));
+        // Each exit face's look, keeping the first face (in facet order)
+        // that holds the probe point; a point on two faces at once (within
+        // margin of an edge) is left to the full test.
+        FaceLook sel = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, false, false, false};
+        int selAx = -1, nIn = 0;
+        bool unsure = false;
+        {
+            const FaceLook l = look_exit_face<0>(c9, x, y, z, a, b, g);
+            unsure = unsure || l.unsure;
+            if (l.inside) {
+                nIn++;
+                sel = l;
+                selAx = 0;
+            }
+        }
+        {
+            const FaceLook l = look_exit_face<1>(c9, x, y, z, a, b, g);
+            unsure = unsure || l.unsure;
+            if (l.inside) {
+                nIn++;
+                if (selAx < 0) {
+                    sel = l;
+                    selAx = 1;
+                }
+            }
+        }
+        {
+            const FaceLook l = look_exit_face<2>(c9, x, y, z, a, b, g);
+            unsure = unsure || l.unsure;
+            if (l.inside) {
+                nIn++;
+                if (selAx < 0) {
+                    sel = l;
+                    selAx = 2;
+                }
+            }
+        }
+        FaceProbe fp = {-1, -1, false, false};
+        if (!unsure && nIn == 1) fp = probe_face(selAx, sel);
+        unsure = unsure || nIn > 1 || fp.unsure;
+        // the lane's candidates, at most two, both on that face
+        const int n = (fp.c1 >= 0) + (fp.c2 >= 0);
+        const int q0 = fp.c1;
+        const bool deep = !unsure && n == 1 && fp.deep;
         if (deep) {
             // the only candidate, its exit point deeper inside it than
             // margin: the reference's containment test passes for it, so
@@ -845,7 +908,7 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
                 nf.d = kHuge;
                 nf.f = (d != kHuge && q0 == 23) ? 22 : 23;
             }
-        } else if (unsure || n > 3) {
+        } else if (unsure) {
             // the reference's test on the 12 facets of the three exit faces
             FindNearest fn = {1e80, -1, -1, -kHuge, 0};
             visit_axis<0>(fn, pl, c9, planeTol, x, y, z, a, b, g);
@@ -854,17 +917,7 @@ __device__ __forceinline__ Nearest nearest_facet(const DevProblem& P, const doub
             nf = fn.result();
         } else {
             FindNearest fn = {1e80, -1, -1, -kHuge, 0};
-            // the second and third candidates in facet order (n <= 3 here)
-            int m = 0, q1 = -1, q2 = -1;
-            const int cs[6] = {f0.c1, f0.c2, f1.c1, f1.c2, f2.c1, f2.c2};
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                if (cs[k] >= 0) {
-                    if (m == 1) q1 = cs[k];
-                    else if (m == 2) q2 = cs[k];
-                    m++;
-                }
-            }
+            const int q1 = fp.c2, q2 = -1;
             // exact tests of the candidates, merged in facet order; every
             // other facet is kHuge, which only sets the largest kHuge index
             bool fin0 = false, fin1 = false, fin2 = false;

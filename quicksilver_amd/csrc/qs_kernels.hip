@@ -994,18 +994,25 @@ struct Lane {
     int ci, cj, ck, group;
 };
 
-// updateTrajectory (CollisionEvent.cc:25-48)
 // updateTrajectory (CollisionEvent.cc:25-45) up to its last step, the new
 // number of mean free paths -1 * log(rngSample) (CollisionEvent.cc:44),
 // which the caller draws (sample_mfp) so that one copy of the log serves
-// every lane that needs a fresh draw.
+// every lane that needs a fresh draw.  kSinCos takes sin and cos of the
+// azimuth from one sincos call (the same two values; it shares the argument
+// reduction, at the cost of registers the 2-wave kernel builds have).
+template <bool kSinCos>
 __device__ __forceinline__ void update_direction_energy(Lane& L, double energy, double angle) {
     L.E = energy;
     double cosTheta = angle;
     double r = rng_sample(L.seed);
     double phi = 2 * 3.14159265 * r;
-    double sinPhi = sin(phi);
-    double cosPhi = cos(phi);
+    double sinPhi, cosPhi;
+    if (kSinCos) {
+        sincos(phi, &sinPhi, &cosPhi);
+    } else {
+        sinPhi = sin(phi);
+        cosPhi = cos(phi);
+    }
     double sinTheta = sqrt((1.0 - (cosTheta * cosTheta)));
     rotate3d(L.a, L.b, L.g, sinTheta, cosTheta, sinPhi, cosPhi);
     double speed = (kLight * sqrt((1.0 - ((kRestMass * kRestMass) /
@@ -1359,6 +1366,7 @@ struct StackRef {
 // Tracks one problem's pass queue until it is drained (queue exhausted,
 // stack empty, every lane idle), then adds the wave's balance counters to
 // the problem's tallies.
+template <bool kSinCos>
 __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T, const QueueIO& io,
                                             const TrackTuning& tune, const StackRef& sr) {
     const int G = P.G;
@@ -1485,7 +1493,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         }
         // updateTrajectory (CollisionEvent.cc:135-140) of scatter survivors
         // and collision records
-        if (upd) update_direction_energy(L, L.E, L.nmfp);
+        if (upd) update_direction_energy<kSinCos>(L, L.E, L.nmfp);
         if (upd || fresh) sample_mfp(L);
         if (upd || ld) {
             L.speed = vlength(L.vx, L.vy, L.vz);
@@ -1796,7 +1804,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
 // (k_track) or, round robin, every problem of a batch with queued particles
 // (k_batch_track).  Secondaries go to the wave's stack and are tracked
 // before the wave takes new queue entries.
-template <class Q>
+template <bool kSinCos, class Q>
 __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune, const WaveStacks& ws) {
     __shared__ uint32_t sTop[4];
     const uint32_t wib = threadIdx.x >> 6;
@@ -1806,7 +1814,7 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune, const
     if (__lane_id() == 0) sTop[wib] = 0;
     do {
         acc.stage();
-        track_queue(acc.prob(), acc.tables(), acc.queue(), tune, sr);
+        track_queue<kSinCos>(acc.prob(), acc.tables(), acc.queue(), tune, sr);
     } while (Q::kMulti && acc.next());
 }
 
@@ -1833,7 +1841,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMinWa
     QueueIO io = {in, *nInPtr, &pc->head, census, &pc->nCensus, censusCap, extra, nExtraOut, extraCap,
                   &pc->error, dc};
     SingleQueue acc = {P, T, io};
-    run_queue(acc, tune, ws);
+    run_queue<(kMinWaves <= 2)>(acc, tune, ws);
 }
 
 constexpr int kFluxBlock = 256;
@@ -2740,7 +2748,7 @@ k_batch_track(BatchSlot* slots, int n, int passKind, TrackTuning tune, WaveStack
     BatchQueue<kStageE> acc = {slots, n, passKind, (int)(wave % (uint32_t)n), sMat[threadIdx.x >> 6],
                                sReact[threadIdx.x >> 6], sE[threadIdx.x >> 6]};
     if (!slots[acc.p].st->active && !acc.next()) return;
-    run_queue(acc, tune, ws);
+    run_queue<(kMinWaves <= 2)>(acc, tune, ws);
 }
 
 // Fluence of every active problem that accumulates it (one block row per

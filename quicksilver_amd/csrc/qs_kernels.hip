@@ -108,9 +108,12 @@ __device__ __forceinline__ uint64_t rng_spawn(uint64_t& parent) {
 // NuclearData::getEnergyGroup (NuclearData.cc:208-228).  The reference
 // returns G for E > eMax, one past the last table row; tables here have G
 // rows, so that case is clamped to G-1.
-__device__ __forceinline__ int energy_group(const double* e, int G, double E, float log2Lo, float perLog2) {
-    if (E <= e[0]) return 0;
-    if (E > e[G]) return G - 1;
+// eLo and eHi are e[0] and e[G], held in registers by callers that search
+// often.
+__device__ __forceinline__ int energy_group(const double* e, int G, double E, double eLo, double eHi,
+                                            float log2Lo, float perLog2) {
+    if (E <= eLo) return 0;
+    if (E > eHi) return G - 1;
     if (E != E) return G - 1;  // what the bisection returns for NaN
     // The bisection's answer is the largest g <= G-1 with e[g] <= E.  The
     // bounds are log-spaced, so a single-precision log2 puts the guess within
@@ -992,6 +995,7 @@ struct Lane {
     double E, w, tc, nmfp;
     uint64_t seed;
     int ci, cj, ck, group;
+    int mat;  // the cell's material, read when the cell changes
 };
 
 // updateTrajectory (CollisionEvent.cc:25-45) up to its last step, the new
@@ -1370,6 +1374,8 @@ template <bool kSinCos>
 __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T, const QueueIO& io,
                                             const TrackTuning& tune, const StackRef& sr) {
     const int G = P.G;
+    // the group table's end bounds, read once: every group search tests them
+    const double eLo = rfld(T.energies[0]), eHi = rfld(T.energies[G]);
     const uint32_t nIn = io.nIn;
     const int lane = __lane_id();
     const uint64_t lt = lanemask_lt();
@@ -1478,6 +1484,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             rec = cell < 0;
             if (rec) cell = ~cell;
             decode_cell(P, cell, L.ci, L.cj, L.ck);
+            L.mat = P.cellMat[cell];
             if (rec) {
                 L.a = L.vx;
                 L.b = L.vy;
@@ -1511,14 +1518,14 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 if (L.tc <= 0.0) L.tc += P.dt;
             }
         }
-        if (grp) L.group = energy_group(T.energies, G, L.E, T.eLog2Lo, T.eGroupsPerLog2);
+        if (grp) L.group = energy_group(T.energies, G, L.E, eLo, eHi, T.eLog2Lo, T.eGroupsPerLog2);
         upd = rec = grp = ld = false;
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
         if (active) {
         // ---- one segment: MC_Segment_Outcome (MC_Segment_Outcome.cc:31-226) ----
         const int cell = L.ci + P.nx * (L.cj + P.ny * L.ck);
-        const int mat = P.cellMat[cell];
+        const int mat = L.mat;
         bool force = false;
         if (L.nmfp < 0.0) {
             force = true;
@@ -1733,6 +1740,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                 L.ci = ni;
                 L.cj = nj;
                 L.ck = nk;
+                // read here, used by the next segment: the load has the rest
+                // of this iteration and the next one's set-up to land
+                L.mat = P.cellMat[ni + P.nx * (nj + P.ny * nk)];
             } else if (((P.boundaryMask >> (2 * face)) & 3) == kEscape) {
                 ev |= kEvEscape;
                 active = false;
@@ -1768,7 +1778,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             // EnergySpectrum::UpdateSpectrum (EnergySpectrum.cc:12-35, called
             // from Tallies.cc:97 on every census particle of every cycle):
             // the particle's getEnergyGroup, which is G above eMax
-            if (P.spectrum) atomicAdd(&P.spectrum[(L.E > T.energies[G]) ? G : L.group], 1ull);
+            if (P.spectrum) atomicAdd(&P.spectrum[(L.E > eHi) ? G : L.group], 1ull);
             ev |= kEvCensus;
             active = false;
         }
@@ -2407,7 +2417,8 @@ __global__ void __launch_bounds__(256) k_fluence(const double* __restrict__ flux
 // (the same search MC_Load_Particle does) scaled to kSortBands.
 
 __device__ __forceinline__ uint32_t sort_key(const DevProblem& P, const Particles& in, uint32_t i) {
-    const int g = energy_group(P.energies, P.G, in.E()[i], P.eLog2Lo, P.eGroupsPerLog2);
+    const int g = energy_group(P.energies, P.G, in.E()[i], P.energies[0], P.energies[P.G], P.eLog2Lo,
+                               P.eGroupsPerLog2);
     return (uint32_t)in.cell()[i] * kSortBands + (uint32_t)(g * kSortBands / P.G);
 }
 

@@ -1662,11 +1662,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             // the multiplicity, then the outgoing (energy, angle) pairs in
             // order, each drawn once.  Pair 0 is the parent's.
             int nOut = 0;
-            double e0 = 0.0, a0 = 0.0;
             if (type == kScatter) {
                 nOut = 1;
-                e0 = L.E * (1.0 - (rng_sample(L.seed) * mi.invMass));  // 1.0 / mass
-                a0 = rng_sample(L.seed) * 2.0 - 1.0;
             } else if (type == kFission) {
                 int n = (int)(mi.nuBar + rng_sample(L.seed));
                 nOut = n;
@@ -1674,11 +1671,20 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
                     atomicOr(io.error, 16u);  // MAX_PRODUCTION_SIZE exceeded
                     nOut = 4;
                 }
-                if (nOut > 0) {
-                    const double rr = rng_sample(L.seed) / 2.0 + 0.5;
+            }
+            // pair 0: a scatter's two draws, or the two after a fission's
+            // multiplicity, drawn once for both kinds of lane
+            double e0 = 0.0, a0 = 0.0;
+            if (nOut > 0) {
+                const double x1 = rng_sample(L.seed);
+                const double x2 = rng_sample(L.seed);
+                if (type == kScatter) {
+                    e0 = L.E * (1.0 - (x1 * mi.invMass));  // 1.0 / mass
+                } else {
+                    const double rr = x1 / 2.0 + 0.5;
                     e0 = (20 * rr * rr);
-                    a0 = rng_sample(L.seed) * 2.0 - 1.0;
                 }
+                a0 = x2 * 2.0 - 1.0;
             }
             ev |= kEvCollision;
             if (type == kScatter) ev |= kEvScatter;

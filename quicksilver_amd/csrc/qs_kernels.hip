@@ -1575,8 +1575,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             fluxAcc += contrib;
         }
         histSeg++;
-        // a census particle's slot, reserved before the collision and
-        // crossing work so that the append's round trip passes under it
+        // a census particle's slot (the append waits for its atomic here;
+        // deferring that wait past the collision work measured slower, the
+        // pending atomic holding up the loads issued after it)
         uint32_t censusSlot = 0;
         if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);
 

@@ -4,7 +4,8 @@
 // expression rounds exactly like the reference's non-FMA x86-64 build; the
 // order of operations inside each expression follows the reference line
 // cited next to it.  Integer RNG streams are bit-exact.  log/sin/cos come
-// from the device math library (within 1 ulp of glibc).
+// from the device math library (within 2 ulp of the host's glibc, checked
+// by the math probe tests); sqrt and division are correctly rounded.
 //
 // Kernels (k_batch_* are the same over every problem of a batch)
 //   k_cycle_setup  carried count, population factor, clears main.cc:96-121

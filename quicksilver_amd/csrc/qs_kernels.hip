@@ -1399,6 +1399,11 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     // iteration
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
     double fluxAcc = 0.0;  // this lane's scalar-flux contributions
+    // Census particles staged at the top end of the wave's stack (entries
+    // scap-1, scap-2, ...), appended to the census buffer when the queue is
+    // drained.  The count must be wave-uniform: it changes only outside the
+    // active lanes' branch, from a ballot.
+    uint32_t nStaged = 0;
 
     for (;;) {
         // ---- refill idle lanes: the wave's stack first (most recent
@@ -1471,7 +1476,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             if (lane == 0) *sTopW = top;
         }
         // room for this iteration's pushes (uniform: decided before any lane diverges)
-        const bool stackOk = top + kStackMargin <= scap;
+        const bool stackOk = top + kStackMargin + nStaged <= scap;
         const uint64_t segLanes = __ballot(active);  // each takes one segment below
         if (segLanes == 0) break;                    // drained
         cSeg += __popcll(segLanes);
@@ -1523,6 +1528,9 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         upd = rec = grp = ld = false;
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
+        // room to stage up to one census particle per lane above the room
+        // this iteration's pushes may take (uniform)
+        const bool cenRoom = top + kStackMargin + nStaged + 64 <= scap;
         if (active) {
         // ---- one segment: MC_Segment_Outcome (MC_Segment_Outcome.cc:31-226) ----
         const int cell = L.ci + P.nx * (L.cj + P.ny * L.ck);
@@ -1576,11 +1584,12 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             fluxAcc += contrib;
         }
         histSeg++;
-        // a census particle's slot (the append waits for its atomic here;
-        // deferring that wait past the collision work measured slower, the
-        // pending atomic holding up the loads issued after it)
+        // A census particle's slot: staged on the wave's stack so that no
+        // atomic round trip stalls the loop, or, when the stack has no room,
+        // appended to the census buffer at once.
         uint32_t censusSlot = 0;
-        if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);
+        if (cenRoom) censusSlot = scap - 1 - (nStaged + (uint32_t)__popcll(__ballot(outcome == 2) & lt));
+        else if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);
 
         if (outcome == 0) {
             // ---- CollisionEvent (CollisionEvent.cc:50-148) ----
@@ -1780,8 +1789,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             }
         } else {
             // ---- census: processedVault->pushParticle (CycleTracking.cc:104-111) ----
-            const uint32_t o = censusSlot;
-            if (o < io.censusCap) store_particle(io.census, o, L, cell);
+            if (cenRoom) store_particle(stack, censusSlot, L, cell);
+            else if (censusSlot < io.censusCap) store_particle(io.census, censusSlot, L, cell);
             else atomicOr(io.error, 1u);
             // EnergySpectrum::UpdateSpectrum (EnergySpectrum.cc:12-35, called
             // from Tallies.cc:97 on every census particle of every cycle):
@@ -1802,9 +1811,29 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         cAbs += __popcll(__ballot(ev & kEvAbsorb));
         cFis += __popcll(__ballot(ev & kEvFission));
         cEsc += __popcll(__ballot(ev & kEvEscape));
-        cCen += __popcll(__ballot(ev & kEvCensus));
+        const uint32_t nCenIt = (uint32_t)__popcll(__ballot(ev & kEvCensus));
+        cCen += nCenIt;
+        if (cenRoom) nStaged += nCenIt;  // the lanes that staged above
         cProd += __popcll(__ballot(nProd > 0)) + __popcll(__ballot(nProd > 1)) +
                  __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
+    }
+    // the staged census particles to the census buffer, one atomic per wave
+    if (nStaged != 0) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(io.nCensus, nStaged);
+        base = __shfl(base, 0);
+        for (uint32_t q = (uint32_t)lane; q < nStaged; q += 64) {
+            const uint32_t o = base + q;
+            if (o < io.censusCap) {
+                const uint32_t i = scap - 1 - q;
+                store_record(io.census, o, stack.x()[i], stack.y()[i], stack.z()[i], stack.vx()[i], stack.vy()[i],
+                             stack.vz()[i], stack.E()[i], stack.w()[i], stack.tc()[i], stack.nmfp()[i],
+                             stack.seed()[i], stack.cell()[i]);
+            } else {
+                atomicOr(io.error, 1u);
+            }
+        }
     }
     flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
     // The cycle's flux sum (Tallies::ScalarFluxSum, Tallies.cc:146-171, the

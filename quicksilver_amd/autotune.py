@@ -44,7 +44,7 @@ SPACE = {
     "waves_per_cu": [4, 8, 12, 16, 24, 32],
     "refill_min": [1, 4, 8, 16, 32],
     "chunk": [16, 32, 64, 128, 256],
-    "stack_entries": [0, 512, 2048],
+    "stack_entries": [0, 512, 1024, 2048],
     "sort_interval": [0, 1, 2, 4],
     "waves_per_simd": [2, 3, 4],
 }

@@ -176,7 +176,12 @@ constexpr uint32_t kStackCap = 512;
 // room that must be left on a stack before an iteration may push to it:
 // one collision per lane, at most 4 outputs each
 constexpr uint32_t kStackMargin = 256;
-inline size_t waveStackBytes(uint32_t nWaves, uint32_t cap) { return (size_t)nWaves * 12 * cap * 8; }
+// census particles a wave holds before appending them to the census buffer
+// (a region of the wave's own after its stack entries)
+constexpr uint32_t kCensusStage = 256;
+// one wave's region: its stack entries, then its census staging entries
+constexpr uint32_t waveRegionEntries(uint32_t cap) { return cap == 0 ? 0u : cap + kCensusStage; }
+inline size_t waveStackBytes(uint32_t nWaves, uint32_t cap) { return (size_t)nWaves * 12 * waveRegionEntries(cap) * 8; }
 constexpr TrackTuning kDefaultTuning = {256, 12, 1, 32, (int)kStackCap, 0, 3, 1};
 
 // ---- batched form: many independent problems per launch ----

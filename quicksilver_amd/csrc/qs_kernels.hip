@@ -1362,6 +1362,44 @@ __device__ __forceinline__ void flush_tallies(DevCounters* dc, uint32_t& cSeg, u
 // global loads after an optimisation barrier.
 typedef const __attribute__((address_space(1))) char* GlobalBytes;
 
+// Appends a wave's n staged census particles (region entries scap ..
+// scap+n-1) to the census buffer: one atomic for all of them, then a copy,
+// the wave's lanes taking one record each in turn.  Each record's fields
+// are reached through one address per buffer stepped by that buffer's
+// stride, as the refill reads them.  Called by all lanes of the wave with a
+// wave-uniform n > 0.
+__device__ __forceinline__ void flush_census_stage(const Particles& stack, uint32_t scap, uint32_t n,
+                                                   const QueueIO& io) {
+    typedef __attribute__((address_space(1))) char* GlobalBytesW;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(io.nCensus, n);
+    base = __shfl(base, 0);
+    for (uint32_t q = lane; q < n; q += 64) {
+        const uint32_t o = base + q;
+        if (o < io.censusCap) {
+            const uint32_t i = scap + q;
+            GlobalBytes sp = (GlobalBytes)stack.base + (uint64_t)i * 8;
+            GlobalBytesW dp = (GlobalBytesW)io.census.base + (uint64_t)o * 8;
+            // the ten doubles and the seed, copied as 64-bit words
+#pragma unroll
+            for (int k = 0; k < 11; k++) {
+                *(__attribute__((address_space(1))) uint64_t*)dp =
+                    *(const __attribute__((address_space(1))) uint64_t*)sp;
+                sp += stack.stride;
+                dp += io.census.stride;
+                asm("" : "+v"(sp), "+v"(dp));  // keep each step an add
+            }
+            // the int32 cell field: its array starts 11 strides in
+            *(__attribute__((address_space(1))) int32_t*)(dp - (uint64_t)o * 4) =
+                *(const __attribute__((address_space(1))) int32_t*)(sp - (uint64_t)i * 4);
+        } else {
+            atomicOr(io.error, 1u);
+        }
+    }
+}
+
 struct StackRef {
     Particles view;
     uint32_t cap;
@@ -1399,10 +1437,11 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
     // iteration
     uint32_t cSeg = 0, cCol = 0, cScat = 0, cAbs = 0, cFis = 0, cProd = 0, cEsc = 0, cCen = 0;
     double fluxAcc = 0.0;  // this lane's scalar-flux contributions
-    // Census particles staged at the top end of the wave's stack (entries
-    // scap-1, scap-2, ...), appended to the census buffer when the queue is
-    // drained.  The count must be wave-uniform: it changes only outside the
-    // active lanes' branch, from a ballot.
+    // Census particles staged in the wave's region after its stack (entries
+    // scap, scap+1, ...), appended to the census buffer with one atomic when
+    // the staging entries run short or the queue is drained.  The count must
+    // be wave-uniform: it changes only outside the active lanes' branch,
+    // from a ballot.
     uint32_t nStaged = 0;
 
     for (;;) {
@@ -1476,9 +1515,13 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             if (lane == 0) *sTopW = top;
         }
         // room for this iteration's pushes (uniform: decided before any lane diverges)
-        const bool stackOk = top + kStackMargin + nStaged <= scap;
+        const bool stackOk = top + kStackMargin <= scap;
         const uint64_t segLanes = __ballot(active);  // each takes one segment below
-        if (segLanes == 0) break;                    // drained
+        if (nStaged + 64 > kCensusStage || (segLanes == 0 && nStaged != 0)) {
+            flush_census_stage(stack, scap, nStaged, io);
+            nStaged = 0;
+        }
+        if (segLanes == 0) break;  // drained
         cSeg += __popcll(segLanes);
         // ---- per-lane work of loaded particles and collision survivors,
         // one copy for all lanes that need it ----
@@ -1528,9 +1571,8 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         upd = rec = grp = ld = false;
         uint32_t ev = 0;  // per-lane event bits of this iteration
         uint32_t nProd = 0;
-        // room to stage up to one census particle per lane above the room
-        // this iteration's pushes may take (uniform)
-        const bool cenRoom = top + kStackMargin + nStaged + 64 <= scap;
+        // room to stage one census particle per lane (uniform)
+        const bool cenRoom = scap != 0 && nStaged + 64 <= kCensusStage;
         if (active) {
         // ---- one segment: MC_Segment_Outcome (MC_Segment_Outcome.cc:31-226) ----
         const int cell = L.ci + P.nx * (L.cj + P.ny * L.ck);
@@ -1588,7 +1630,7 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         // atomic round trip stalls the loop, or, when the stack has no room,
         // appended to the census buffer at once.
         uint32_t censusSlot = 0;
-        if (cenRoom) censusSlot = scap - 1 - (nStaged + (uint32_t)__popcll(__ballot(outcome == 2) & lt));
+        if (cenRoom) censusSlot = scap + nStaged + (uint32_t)__popcll(__ballot(outcome == 2) & lt);
         else if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);
 
         if (outcome == 0) {
@@ -1817,24 +1859,6 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
         cProd += __popcll(__ballot(nProd > 0)) + __popcll(__ballot(nProd > 1)) +
                  __popcll(__ballot(nProd > 2)) + __popcll(__ballot(nProd > 3));
     }
-    // the staged census particles to the census buffer, one atomic per wave
-    if (nStaged != 0) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(io.nCensus, nStaged);
-        base = __shfl(base, 0);
-        for (uint32_t q = (uint32_t)lane; q < nStaged; q += 64) {
-            const uint32_t o = base + q;
-            if (o < io.censusCap) {
-                const uint32_t i = scap - 1 - q;
-                store_record(io.census, o, stack.x()[i], stack.y()[i], stack.z()[i], stack.vx()[i], stack.vy()[i],
-                             stack.vz()[i], stack.E()[i], stack.w()[i], stack.tc()[i], stack.nmfp()[i],
-                             stack.seed()[i], stack.cell()[i]);
-            } else {
-                atomicOr(io.error, 1u);
-            }
-        }
-    }
     flush_tallies(io.dc, cSeg, cCol, cScat, cAbs, cFis, cProd, cEsc, cCen);
     // The cycle's flux sum (Tallies::ScalarFluxSum, Tallies.cc:146-171, the
     // sum over all bins) as the sum of the contributions that filled the
@@ -1857,7 +1881,8 @@ __device__ __forceinline__ void run_queue(Q& acc, const TrackTuning& tune, const
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t gw = (uint32_t)__builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const uint32_t scap = (gw < ws.nWaves) ? ws.cap : 0u;
-    const StackRef sr = {{ws.base + (size_t)gw * 12 * scap * 8, (uint64_t)scap * 8}, scap, &sTop[wib]};
+    const uint32_t region = waveRegionEntries(scap);  // stack, then census staging
+    const StackRef sr = {{ws.base + (size_t)gw * 12 * region * 8, (uint64_t)region * 8}, scap, &sTop[wib]};
     if (__lane_id() == 0) sTop[wib] = 0;
     do {
         acc.stage();

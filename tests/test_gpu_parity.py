@@ -76,6 +76,24 @@ def test_launch_shape_does_not_change_tallies(name, shape):
 
 
 @pytest.mark.gpu
+def test_census_staging_flushed_within_a_pass():
+    """A wave stages its census particles (256 entries after its stack) and
+    appends them to the census buffer when fewer than 64 entries are left,
+    or when its queue is drained.  With 256 waves (64-thread workgroups, one
+    wave per CU), checker_small's 1.5e5 census particles per cycle give each
+    wave about 580, so every wave flushes within the pass as well as at its
+    end; the next cycle tracks what those flushes wrote.  Every cycle must
+    match the reference's tallies."""
+    print("[parity] deck=checker_small shape=one wave per CU (census flushed within the pass)", flush=True)
+    g = load_golden("checker_small")
+    assert min(c["census"] for c in g["cycles"]) > 256 * 256  # > 256 per wave
+    with q.Problem.from_file(g["deck_path"]) as p:
+        p.set_tuning(block_size=64, waves_per_cu=1)
+        out = p.run()
+    _check_against(g, out, "one wave per CU")
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["tiny_cube", "nested_small"])
 def test_flux_per_bin_and_census_particles(name):
     """Per-(cell, group) flux of the last cycle and every census particle

@@ -246,7 +246,10 @@ void destroyProblem(qsg_problem_s* p) {
     if (!p) return;
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    for (void* a : p->allocs) (void)hipFree(a);
+    // no work on the device may still use a block handed back for reuse
+    // (hipFree would have waited for the device too)
+    if (!p->allocs.empty()) (void)hipDeviceSynchronize();
+    for (const auto& a : p->allocs) qsgi::deviceRelease(p->device, a.first, a.second);
     p->stacks.release();
     for (qs::CycleBlock* h : p->hcb) qsgi::releasePinned(h);
     for (hipEvent_t e : {p->ev0, p->ev1, p->evI0, p->evI1, p->evF0, p->evF1})
@@ -696,6 +699,62 @@ void qsgi::releasePinned(void* p) {
     if (!p) return;
     std::lock_guard<std::mutex> lock(g_poolMutex);
     g_pinned.push_back(static_cast<char*>(p));
+}
+
+namespace {
+struct DeviceCache {
+    std::multimap<size_t, void*> idle;  // block size -> block
+    size_t idleBytes = 0;
+};
+std::map<int, DeviceCache> g_devCache;  // guarded by g_poolMutex
+
+void releaseIdle(DeviceCache& c) {
+    for (const auto& b : c.idle) (void)hipFree(b.second);
+    c.idle.clear();
+    c.idleBytes = 0;
+}
+}  // namespace
+
+void* qsgi::deviceAlloc(int device, size_t bytes, size_t* blockBytes) {
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        DeviceCache& c = g_devCache[device];
+        auto it = c.idle.lower_bound(bytes);
+        if (it != c.idle.end() && it->first / 3 * 2 <= bytes) {
+            void* p = it->second;
+            *blockBytes = it->first;
+            c.idleBytes -= it->first;
+            c.idle.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lock(g_poolMutex);
+            releaseIdle(g_devCache[device]);
+        }
+        e = hipMalloc(&p, bytes);
+    }
+    hipCheck(e, "hipMalloc");
+    *blockBytes = bytes;
+    return p;
+}
+
+void qsgi::deviceRelease(int device, void* p, size_t blockBytes) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lock(g_poolMutex);
+        DeviceCache& c = g_devCache[device];
+        if (c.idleBytes + blockBytes <= kDeviceCacheBytes) {
+            c.idle.emplace(blockBytes, p);
+            c.idleBytes += blockBytes;
+            return;
+        }
+    }
+    (void)hipFree(p);
 }
 
 void qsgi::StackPool::release() {

@@ -67,6 +67,21 @@ constexpr size_t kPinnedBlock = 256;     // bytes per pinned block
 void* acquirePinned(size_t bytes);       // bytes <= kPinnedBlock
 void releasePinned(void* p);
 
+// Device memory blocks are handed back when a problem is released and
+// reused by the next allocation of a similar size on the same device:
+// hipMalloc and hipFree cost tens of microseconds each, and a problem makes
+// about twenty allocations, which a served stream of small problems pays on
+// its critical path.  A cached block serves a request of at least two
+// thirds of its size; at most kDeviceCacheBytes of idle blocks are kept per
+// device, and an allocation that fails releases the device's idle blocks
+// and tries once more.  Thread-safe.  Whoever hands a block back makes sure
+// that no work still uses it.  A reused block keeps its old contents.
+constexpr size_t kDeviceCacheBytes = 64ull << 30;
+// `bytes` on the current device; *blockBytes receives the block's size,
+// which deviceRelease takes back with it.
+void* deviceAlloc(int device, size_t bytes, size_t* blockBytes);
+void deviceRelease(int device, void* p, size_t blockBytes);
+
 // Device memory for the per-wave secondary stacks of tracking launches,
 // grown on demand; the owner's stream is synchronised before memory that a
 // launch may still use is freed.
@@ -88,7 +103,7 @@ struct qsg_problem_s {
     qs::Parameters params;
     qs::ProblemTables tables;
     qs::DevProblem dp = {};
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;  // device blocks and their sizes
     size_t deviceBytes = 0;
 
     PBuf proc[2], pop, extra[2];
@@ -141,11 +156,11 @@ struct qsg_problem_s {
 
     template <typename T>
     T* dalloc(size_t count) {
-        void* p = nullptr;
         size_t bytes = count * sizeof(T);
         if (bytes == 0) bytes = 8;
-        qsgi::hipCheck(hipMalloc(&p, bytes), "hipMalloc");
-        allocs.push_back(p);
+        size_t block = 0;
+        void* p = qsgi::deviceAlloc(device, bytes, &block);
+        allocs.emplace_back(p, block);
         deviceBytes += bytes;
         return static_cast<T*>(p);
     }

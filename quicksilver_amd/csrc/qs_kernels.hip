@@ -1626,9 +1626,10 @@ __device__ __forceinline__ void track_queue(const DevProblem& P, const Tables& T
             fluxAcc += contrib;
         }
         histSeg++;
-        // A census particle's slot: staged on the wave's stack so that no
-        // atomic round trip stalls the loop, or, when the stack has no room,
-        // appended to the census buffer at once.
+        // A census particle's slot: staged in the wave's region after its
+        // stack, so that no atomic round trip stalls the loop, or, for a
+        // wave without a region (no stacks), appended to the census buffer
+        // at once.
         uint32_t censusSlot = 0;
         if (cenRoom) censusSlot = scap + nStaged + (uint32_t)__popcll(__ballot(outcome == 2) & lt);
         else if (outcome == 2) censusSlot = wave_append(io.nCensus, 1u);

@@ -3,9 +3,12 @@
 // reference's x86-64 (no FMA) build.
 #include "qs_problem.h"
 
+#include <algorithm>
 #include <cmath>
+#include <exception>
 #include <map>
 #include <stdexcept>
+#include <thread>
 
 namespace qs {
 
@@ -230,7 +233,20 @@ ProblemTables buildProblem(const Parameters& params) {
     T.cellSourceOffset.assign(nCells + 1, 0);
 
     std::vector<double> cellWeight(nCells, 0.0);
-    for (int k = 0; k < sp.nz; k++)
+    // each geometry's material index (-1: a name no material has), and the
+    // index a cell no geometry covers would take (the empty name's)
+    std::vector<int> geomMat;
+    for (const GeometryParameters& g : params.geometryParams) {
+        auto mi = matIndex.find(g.materialName);
+        geomMat.push_back(mi == matIndex.end() ? -1 : mi->second);
+    }
+    const auto emptyMi = matIndex.find(std::string());
+    const int uncovered = emptyMi == matIndex.end() ? -1 : emptyMi->second;
+    // Cells are independent: each writes its own entries only, so slabs of k
+    // are built on several threads for large meshes (the results do not
+    // depend on the split).
+    auto buildSlab = [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++)
         for (int j = 0; j < sp.ny; j++)
             for (int i = 0; i < sp.nx; i++) {
                 const size_t c = (size_t)i + (size_t)sp.nx * ((size_t)j + (size_t)sp.ny * k);
@@ -319,15 +335,35 @@ ProblemTables buildProblem(const Parameters& params) {
                 if (ix != i || iy != j || iz != k) throw std::runtime_error("cell id mismatch");
 
                 // material: last geometry containing the cell position wins
-                std::string matName;
-                for (const GeometryParameters& g : params.geometryParams)
-                    if (isInside(g, pos)) matName = g.materialName;
-                auto mi = matIndex.find(matName);
-                if (mi == matIndex.end())
-                    throw std::runtime_error("cell centre not covered by a geometry with a known material");
-                T.cellMaterial[c] = mi->second;
-                cellWeight[c] = volume * T.materials[mi->second].sourceRate * sp.dt;
+                int m = uncovered;
+                for (size_t gi = 0; gi < params.geometryParams.size(); gi++)
+                    if (isInside(params.geometryParams[gi], pos)) m = geomMat[gi];
+                if (m < 0) throw std::runtime_error("cell centre not covered by a geometry with a known material");
+                T.cellMaterial[c] = m;
+                cellWeight[c] = volume * T.materials[m].sourceRate * sp.dt;
             }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nThreads = (int)std::min<long long>({(long long)(hw ? hw : 1), 8LL, (long long)sp.nz, nCells / 2048 + 1});
+    if (nThreads <= 1) {
+        buildSlab(0, sp.nz);
+    } else {
+        std::vector<std::thread> pool;
+        std::vector<std::exception_ptr> errs(nThreads);
+        for (int t = 0; t < nThreads; t++) {
+            const int k0 = (int)((long long)sp.nz * t / nThreads), k1 = (int)((long long)sp.nz * (t + 1) / nThreads);
+            pool.emplace_back([&, t, k0, k1] {
+                try {
+                    buildSlab(k0, k1);
+                } catch (...) {
+                    errs[t] = std::current_exception();
+                }
+            });
+        }
+        for (std::thread& th : pool) th.join();
+        for (const std::exception_ptr& e : errs)
+            if (e) std::rethrow_exception(e);
+    }
 
     // ---- source weights (MC_SourceNow.cc:45-66), cell order = gid order ----
     double totalWeight = 0;

@@ -94,6 +94,26 @@ def test_census_staging_flushed_within_a_pass():
 
 
 @pytest.mark.gpu
+def test_reused_device_blocks_do_not_change_tallies():
+    """A released problem's device buffers are kept and handed to the next
+    problem of a similar size, with their old contents.  Problems built
+    one after another on reused blocks (a deck, another deck, the first
+    again, and the first beside a second copy of itself) must each match
+    the reference's tallies."""
+    print("[parity] decks built on reused device blocks", flush=True)
+    gs = {n: load_golden(n) for n in ("nested_small", "small_cube_short")}
+    for name in ("nested_small", "small_cube_short", "nested_small"):
+        with q.Problem.from_file(gs[name]["deck_path"]) as p:
+            out = p.run()
+        _check_against(gs[name], out, f"{name} on reused blocks")
+    with q.Problem.from_file(gs["nested_small"]["deck_path"]) as a, \
+            q.Problem.from_file(gs["nested_small"]["deck_path"]) as b:
+        out_a, out_b = a.run(), b.run()
+    _check_against(gs["nested_small"], out_a, "first of two")
+    _check_against(gs["nested_small"], out_b, "second of two")
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["tiny_cube", "nested_small"])
 def test_flux_per_bin_and_census_particles(name):
     """Per-(cell, group) flux of the last cycle and every census particle
